@@ -1,0 +1,163 @@
+/* lapack_amd.h -- C ABI of the MI355X (gfx950) LU solve library.
+ *
+ * Three layers, all plain C types (no HIP or torch types in signatures;
+ * streams are passed as void* holding a hipStream_t, NULL = the library's
+ * own stream for the device current on the calling thread):
+ *
+ *  1. Fortran LAPACK drop-ins: the exact symbols and calling convention of
+ *     the reference (all arguments by reference, CHARACTER lengths appended
+ *     as size_t), prototypes as in LAPACKE/include/lapack.h.  Column-major,
+ *     in place, IPIV a 1-based interchange sequence.  Pointers may be host
+ *     or device memory (detected per pointer); every call returns with the
+ *     results in place.  Illegal arguments call XERBLA(name, -INFO) exactly
+ *     as the reference does; xerbla_ here is a weak symbol, so a program's
+ *     own XERBLA replaces it at link time (TESTING/LIN/xerbla.f:74-118).
+ *
+ *  2. Value-argument C entry points lapack_amd_*: same semantics, arguments
+ *     by value, INFO returned (negative = illegal argument number, positive =
+ *     first exact zero pivot, below -1000 = a library/HIP failure, see
+ *     LA_ERR_*).  No XERBLA call.
+ *
+ *  3. Device entry points lapack_amd_*_async: device pointers only, enqueued
+ *     on the given stream and returning at once; INFO is written to a device
+ *     int.  The headline measurements use these (inputs resident in HBM).
+ */
+#ifndef LAPACK_AMD_H
+#define LAPACK_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library error codes (INFO values below every LAPACK argument code) -- */
+#define LA_ERR_HIP (-1001)      /* a HIP runtime call failed                   */
+#define LA_ERR_NOMEM (-1002)    /* device allocation failed                    */
+#define LA_ERR_SYNC (-1003)     /* a cooperative panel kernel's barrier timed out */
+#define LA_ERR_NODEVICE (-1004) /* no GPU visible                              */
+#define LA_ERR_LIMIT (-1005)    /* a size beyond what this build supports      */
+
+/* ======================= 1. Fortran LAPACK drop-ins ======================= */
+
+/* DGETRF: A = P*L*U, blocked right-looking with partial pivoting.
+ * Replaces SRC/dgetrf.f:105 (prototype LAPACKE/include/lapack.h:4349-4353). */
+void dgetrf_(const int* m, const int* n, double* a, const int* lda, int* ipiv, int* info);
+
+/* DGETRF2: the recursive panel routine, same contract as DGETRF.
+ * Replaces SRC/dgetrf2.f:112 (LAPACKE/include/lapack.h:4373). */
+void dgetrf2_(const int* m, const int* n, double* a, const int* lda, int* ipiv, int* info);
+
+/* DGETRS: solve A*X = B or A**T*X = B with the factors of DGETRF.
+ * Replaces SRC/dgetrs.f:118 (LAPACKE/include/lapack.h:4436-4450). */
+void dgetrs_(const char* trans, const int* n, const int* nrhs, const double* a, const int* lda,
+             const int* ipiv, double* b, const int* ldb, int* info, size_t trans_len);
+
+/* DGESV: DGETRF then DGETRS('N').  Replaces SRC/dgesv.f:148. */
+void dgesv_(const int* n, const int* nrhs, double* a, const int* lda, int* ipiv, double* b,
+            const int* ldb, int* info);
+
+/* DLASWP: row interchanges.  Replaces SRC/dlaswp.f:112 (LAPACKE/include/lapack.h:11326-11328). */
+void dlaswp_(const int* n, double* a, const int* lda, const int* k1, const int* k2, const int* ipiv,
+             const int* incx);
+
+/* SGETRF / SGETRS: the single-precision twins (SRC/sgetrf.f, SRC/sgetrs.f). */
+void sgetrf_(const int* m, const int* n, float* a, const int* lda, int* ipiv, int* info);
+void sgetrs_(const char* trans, const int* n, const int* nrhs, const float* a, const int* lda,
+             const int* ipiv, float* b, const int* ldb, int* info, size_t trans_len);
+
+/* DSGESV: FP32 LU + FP64 iterative refinement, FP64 fallback.
+ * Replaces SRC/dsgesv.f:191 (LAPACKE/include/lapack.h:3727-3735).  WORK
+ * (N*NRHS doubles) and SWORK (N*(N+NRHS) floats) are accepted for ABI
+ * compatibility; the GPU solver keeps its own device workspace. */
+void dsgesv_(const int* n, const int* nrhs, double* a, const int* lda, int* ipiv, const double* b,
+             const int* ldb, double* x, const int* ldx, double* work, float* swork, int* iter,
+             int* info);
+
+/* DGEMM: C <- alpha*op(A)*op(B) + beta*C on the FP64 matrix cores.
+ * Replaces BLAS/SRC/dgemm.f:213. */
+void dgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k,
+            const double* alpha, const double* a, const int* lda, const double* b, const int* ldb,
+            const double* beta, double* c, const int* ldc, size_t transa_len, size_t transb_len);
+
+/* XERBLA: weak default that reproduces SRC/xerbla.f:86-88 (message, then
+ * STOP) unless recording mode is on (lapack_amd_xerbla_record). */
+void xerbla_(const char* srname, const int* info, size_t srname_len);
+
+/* ===================== 2. value-argument C entry points =================== */
+
+int lapack_amd_dgetrf(int m, int n, double* a, int lda, int* ipiv);
+int lapack_amd_dgetrs(char trans, int n, int nrhs, const double* a, int lda, const int* ipiv, double* b,
+                      int ldb);
+int lapack_amd_dgesv(int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb);
+int lapack_amd_dlaswp(int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx);
+int lapack_amd_sgetrf(int m, int n, float* a, int lda, int* ipiv);
+int lapack_amd_sgetrs(char trans, int n, int nrhs, const float* a, int lda, const int* ipiv, float* b,
+                      int ldb);
+int lapack_amd_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double* a, int lda,
+                     const double* b, int ldb, double beta, double* c, int ldc);
+/* DSGESV semantics (SRC/dsgesv.f:259-427); *iter receives ITER. */
+int lapack_amd_dsgesv(int n, int nrhs, double* a, int lda, int* ipiv, const double* b, int ldb, double* x,
+                      int ldx, int* iter);
+
+/* ========================= 3. device entry points ========================= */
+
+int lapack_amd_dgetrf_async(int m, int n, double* d_a, int lda, int* d_ipiv, int* d_info, void* stream);
+int lapack_amd_dgetrs_async(char trans, int n, int nrhs, const double* d_a, int lda, const int* d_ipiv,
+                            double* d_b, int ldb, void* stream);
+int lapack_amd_sgetrf_async(int m, int n, float* d_a, int lda, int* d_ipiv, int* d_info, void* stream);
+int lapack_amd_sgetrs_async(char trans, int n, int nrhs, const float* d_a, int lda, const int* d_ipiv,
+                            float* d_b, int ldb, void* stream);
+int lapack_amd_dlaswp_async(int n, double* d_a, int lda, int k1, int k2, const int* d_ipiv, int incx,
+                            void* stream);
+int lapack_amd_dgemm_async(char transa, char transb, int m, int n, int k, double alpha, const double* d_a,
+                           int lda, const double* d_b, int ldb, double beta, double* d_c, int ldc, void* stream);
+int lapack_amd_sgemm_async(char transa, char transb, int m, int n, int k, float alpha, const float* d_a,
+                           int lda, const float* d_b, int ldb, float beta, float* d_c, int ldc, void* stream);
+
+/* ============================ tuning and tools ============================ */
+
+/* Block size NB of the outer loop (the ILAENV(1,'DGETRF') value, reference
+ * default 64 at SRC/ilaenv.f:289-295; this build's default 256) and panel
+ * leaf width (<= 64).  0 restores the default.  The analogue of the test
+ * suite's XLAENV hook (TESTING/LIN/xlaenv.f:77). */
+void lapack_amd_set_tuning(int nb, int leaf);
+void lapack_amd_get_tuning(int* nb, int* leaf);
+
+/* XERBLA recording mode (the role of TESTING/LIN/xerbla.f): when on, the
+ * default xerbla_ records the routine name and parameter number and
+ * returns instead of stopping. */
+void lapack_amd_xerbla_record(int on);
+int lapack_amd_xerbla_last(char* name_out, int cap);
+int lapack_amd_xerbla_calls(void);
+
+/* Per-phase GPU time of getrf/getrs, accumulated over calls while enabled
+ * (hipEvents on the call's stream). */
+enum {
+  LA_PHASE_TRANSPOSE = 0,
+  LA_PHASE_PANEL = 1,
+  LA_PHASE_LASWP = 2,
+  LA_PHASE_TRSM = 3,
+  LA_PHASE_GEMM_PANEL = 4,
+  LA_PHASE_GEMM = 5,
+  LA_PHASE_GETRS = 6,
+  LA_PHASE_COUNT = 7
+};
+void lapack_amd_phase_enable(int on);
+void lapack_amd_phase_reset(void);
+void lapack_amd_phase_read(double* ms, long* calls, double* flops);
+
+/* Counter-based generator of seeded test matrices (the role DLARNV plays in
+ * the reference tests, SRC/dlarnv.f:94-172, with an algorithm of its own):
+ * x[k] for k in [0, n) is a pure function of (seed, offset + k), so any
+ * block of a matrix can be produced independently and bit-identically on
+ * any host.  idist: 1 = uniform(0,1), 2 = uniform(-1,1), 3 = normal(0,1). */
+void lapack_amd_dlarnv(int idist, unsigned long long seed, unsigned long long offset, long long n, double* x);
+
+/* Library version string. */
+const char* lapack_amd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LAPACK_AMD_H */

@@ -1,0 +1,242 @@
+"""lapack_amd -- LAPACK's dense LU solve path on one MI355X (gfx950).
+
+Python mirror of the reference interface over the C ABI in
+include/lapack_amd.h.  The functions take the reference's arguments, in the
+reference's order, with the same meaning (SRC/dgetrf.f:105, SRC/dgetrs.f:118,
+SRC/dgesv.f:148, SRC/dlaswp.f:112, BLAS/SRC/dgemm.f:213): arrays are
+column-major buffers (numpy arrays, or torch tensors / raw integers holding a
+device address), IPIV is 1-based, and INFO is returned.
+
+There is no CPU fallback: if liblapack_amd.so is missing or no GPU is present
+the calls fail loudly (ImportError at load, or INFO = LA_ERR_NODEVICE).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblapack_amd.so")
+
+LA_ERR_HIP = -1001
+LA_ERR_NOMEM = -1002
+LA_ERR_SYNC = -1003
+LA_ERR_NODEVICE = -1004
+LA_ERR_LIMIT = -1005
+
+PHASES = ("transpose", "panel", "laswp", "trsm", "gemm_panel", "gemm", "getrs")
+
+_lib = None
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    """The loaded liblapack_amd.so (built by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(the GPU library has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    vp = ctypes.c_void_p
+    i = ctypes.c_int
+    d = ctypes.c_double
+    f = ctypes.c_float
+    ch = ctypes.c_char
+    sig = {
+        "lapack_amd_dgetrf": (i, [i, i, vp, i, vp]),
+        "lapack_amd_sgetrf": (i, [i, i, vp, i, vp]),
+        "lapack_amd_dgetrs": (i, [ch, i, i, vp, i, vp, vp, i]),
+        "lapack_amd_sgetrs": (i, [ch, i, i, vp, i, vp, vp, i]),
+        "lapack_amd_dgesv": (i, [i, i, vp, i, vp, vp, i]),
+        "lapack_amd_dlaswp": (i, [i, vp, i, i, i, vp, i]),
+        "lapack_amd_dgemm": (i, [ch, ch, i, i, i, d, vp, i, vp, i, d, vp, i]),
+        "lapack_amd_dgetrf_async": (i, [i, i, vp, i, vp, vp, vp]),
+        "lapack_amd_sgetrf_async": (i, [i, i, vp, i, vp, vp, vp]),
+        "lapack_amd_dgetrs_async": (i, [ch, i, i, vp, i, vp, vp, i, vp]),
+        "lapack_amd_sgetrs_async": (i, [ch, i, i, vp, i, vp, vp, i, vp]),
+        "lapack_amd_dlaswp_async": (i, [i, vp, i, i, i, vp, i, vp]),
+        "lapack_amd_dgemm_async": (i, [ch, ch, i, i, i, d, vp, i, vp, i, d, vp, i, vp]),
+        "lapack_amd_sgemm_async": (i, [ch, ch, i, i, i, f, vp, i, vp, i, f, vp, i, vp]),
+        "lapack_amd_set_tuning": (None, [i, i]),
+        "lapack_amd_get_tuning": (None, [_c_int_p, _c_int_p]),
+        "lapack_amd_xerbla_record": (None, [i]),
+        "lapack_amd_xerbla_last": (i, [ctypes.c_char_p, i]),
+        "lapack_amd_xerbla_calls": (i, []),
+        "lapack_amd_phase_enable": (None, [i]),
+        "lapack_amd_phase_reset": (None, []),
+        "lapack_amd_phase_read": (None, [_c_dbl_p, ctypes.POINTER(ctypes.c_long), _c_dbl_p]),
+        "lapack_amd_dlarnv": (None, [i, ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_longlong, vp]),
+        "lapack_amd_version": (ctypes.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def _ptr(x):
+    """Address of a buffer: numpy array, torch tensor, or an integer address."""
+    if x is None:
+        return None
+    if isinstance(x, (int, np.integer)):
+        return int(x)
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    raise TypeError(f"cannot take the address of {type(x)}")
+
+
+def _stream(s):
+    if s is None:
+        return None
+    if isinstance(s, int):
+        return s
+    if hasattr(s, "cuda_stream"):
+        return s.cuda_stream
+    raise TypeError(f"not a stream: {type(s)}")
+
+
+def _c(ch):
+    return ch.encode() if isinstance(ch, str) else ch
+
+
+# ---- reference-shaped entry points (synchronous, host or device buffers) ----
+
+def dgetrf(m, n, a, lda, ipiv):
+    """DGETRF(M, N, A, LDA, IPIV, INFO): returns INFO (SRC/dgetrf.f:105)."""
+    return lib().lapack_amd_dgetrf(m, n, _ptr(a), lda, _ptr(ipiv))
+
+
+def sgetrf(m, n, a, lda, ipiv):
+    """SGETRF (SRC/sgetrf.f)."""
+    return lib().lapack_amd_sgetrf(m, n, _ptr(a), lda, _ptr(ipiv))
+
+
+def dgetrs(trans, n, nrhs, a, lda, ipiv, b, ldb):
+    """DGETRS(TRANS, N, NRHS, A, LDA, IPIV, B, LDB, INFO) (SRC/dgetrs.f:118)."""
+    return lib().lapack_amd_dgetrs(_c(trans), n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb)
+
+
+def sgetrs(trans, n, nrhs, a, lda, ipiv, b, ldb):
+    """SGETRS (SRC/sgetrs.f)."""
+    return lib().lapack_amd_sgetrs(_c(trans), n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb)
+
+
+def dgesv(n, nrhs, a, lda, ipiv, b, ldb):
+    """DGESV(N, NRHS, A, LDA, IPIV, B, LDB, INFO) (SRC/dgesv.f:148)."""
+    return lib().lapack_amd_dgesv(n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb)
+
+
+def dlaswp(n, a, lda, k1, k2, ipiv, incx):
+    """DLASWP(N, A, LDA, K1, K2, IPIV, INCX) (SRC/dlaswp.f:112)."""
+    return lib().lapack_amd_dlaswp(n, _ptr(a), lda, k1, k2, _ptr(ipiv), incx)
+
+
+def dgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """DGEMM (BLAS/SRC/dgemm.f:213); returns 0 or -(parameter number)."""
+    return lib().lapack_amd_dgemm(_c(transa), _c(transb), m, n, k, float(alpha), _ptr(a), lda, _ptr(b), ldb,
+                                  float(beta), _ptr(c), ldc)
+
+
+# ---- device entry points (asynchronous on a stream) --------------------------
+
+def dgetrf_async(m, n, d_a, lda, d_ipiv, d_info, stream=None):
+    return lib().lapack_amd_dgetrf_async(m, n, _ptr(d_a), lda, _ptr(d_ipiv), _ptr(d_info), _stream(stream))
+
+
+def sgetrf_async(m, n, d_a, lda, d_ipiv, d_info, stream=None):
+    return lib().lapack_amd_sgetrf_async(m, n, _ptr(d_a), lda, _ptr(d_ipiv), _ptr(d_info), _stream(stream))
+
+
+def dgetrs_async(trans, n, nrhs, d_a, lda, d_ipiv, d_b, ldb, stream=None):
+    return lib().lapack_amd_dgetrs_async(_c(trans), n, nrhs, _ptr(d_a), lda, _ptr(d_ipiv), _ptr(d_b), ldb,
+                                         _stream(stream))
+
+
+def sgetrs_async(trans, n, nrhs, d_a, lda, d_ipiv, d_b, ldb, stream=None):
+    return lib().lapack_amd_sgetrs_async(_c(trans), n, nrhs, _ptr(d_a), lda, _ptr(d_ipiv), _ptr(d_b), ldb,
+                                         _stream(stream))
+
+
+def dlaswp_async(n, d_a, lda, k1, k2, d_ipiv, incx, stream=None):
+    return lib().lapack_amd_dlaswp_async(n, _ptr(d_a), lda, k1, k2, _ptr(d_ipiv), incx, _stream(stream))
+
+
+def dgemm_async(transa, transb, m, n, k, alpha, d_a, lda, d_b, ldb, beta, d_c, ldc, stream=None):
+    return lib().lapack_amd_dgemm_async(_c(transa), _c(transb), m, n, k, float(alpha), _ptr(d_a), lda, _ptr(d_b),
+                                        ldb, float(beta), _ptr(d_c), ldc, _stream(stream))
+
+
+def sgemm_async(transa, transb, m, n, k, alpha, d_a, lda, d_b, ldb, beta, d_c, ldc, stream=None):
+    return lib().lapack_amd_sgemm_async(_c(transa), _c(transb), m, n, k, float(alpha), _ptr(d_a), lda, _ptr(d_b),
+                                        ldb, float(beta), _ptr(d_c), ldc, _stream(stream))
+
+
+# ---- tuning, error hook, profiling, generator -------------------------------
+
+def set_tuning(nb=0, leaf=0):
+    """Outer block size NB and panel leaf width (0 = default); XLAENV analogue."""
+    lib().lapack_amd_set_tuning(int(nb), int(leaf))
+
+
+def get_tuning():
+    nb, leaf = ctypes.c_int(), ctypes.c_int()
+    lib().lapack_amd_get_tuning(ctypes.byref(nb), ctypes.byref(leaf))
+    return nb.value, leaf.value
+
+
+def xerbla_record(on=True):
+    lib().lapack_amd_xerbla_record(1 if on else 0)
+
+
+def xerbla_last():
+    """(routine name, parameter number, number of calls) of the last XERBLA call."""
+    buf = ctypes.create_string_buffer(32)
+    info = lib().lapack_amd_xerbla_last(buf, 32)
+    return buf.value.decode(), info, lib().lapack_amd_xerbla_calls()
+
+
+def phase_enable(on=True):
+    lib().lapack_amd_phase_enable(1 if on else 0)
+
+
+def phase_reset():
+    lib().lapack_amd_phase_reset()
+
+
+def phase_read():
+    """{phase: (ms, calls, flops)} accumulated since the last reset."""
+    n = len(PHASES)
+    ms = (ctypes.c_double * n)()
+    calls = (ctypes.c_long * n)()
+    fl = (ctypes.c_double * n)()
+    lib().lapack_amd_phase_read(ms, calls, fl)
+    return {p: (ms[i], calls[i], fl[i]) for i, p in enumerate(PHASES)}
+
+
+def dlarnv(idist, seed, n, offset=0, out=None):
+    """Counter-based seeded generator: idist 1 U(0,1), 2 U(-1,1), 3 N(0,1)."""
+    if out is None:
+        out = np.empty(int(n), dtype=np.float64)
+    lib().lapack_amd_dlarnv(int(idist), int(seed), int(offset), int(n), out.ctypes.data)
+    return out
+
+
+def matrix(idist, seed, m, n, offset=0):
+    """m x n Fortran-ordered matrix from dlarnv, element (i, j) at index offset + i + j*m."""
+    return dlarnv(idist, seed, m * n, offset).reshape((n, m)).T
+
+
+def version():
+    return lib().lapack_amd_version().decode()

@@ -1,0 +1,473 @@
+// capi.cpp -- the C ABI of include/lapack_amd.h: Fortran LAPACK drop-ins,
+// value-argument entry points and device (async) entry points.
+//
+// Argument checking follows the reference routine by routine (order and
+// parameter numbers): DGETRF SRC/dgetrf.f:144-155, DGETRF2 SRC/dgetrf2.f:152-163,
+// DGETRS SRC/dgetrs.f:157-174, DGESV SRC/dgesv.f:148-162, DGEMM
+// BLAS/SRC/dgemm.f:271-294, DLASWP none (SRC/dlaswp.f:138-150).  The
+// Fortran symbols call XERBLA with the routine name and -INFO and return,
+// like the reference; the lapack_amd_* entry points return -INFO instead.
+#include "la_common.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/lapack_amd.h"
+
+namespace la {
+struct DevCtx;
+template <class T> int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t s);
+template <class T>
+int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* ipiv, T* B, long ldb, hipStream_t s);
+template <class T>
+int laswp_device(int ncols, T* A, long lda, int nrows, int k1, int k2, const int* ipiv, int incx, hipStream_t s);
+void set_tuning(int nb, int leaf);
+int tune_nb();
+int tune_leaf();
+DevCtx* ctx_for_current();
+hipStream_t ctx_stream(DevCtx* c);
+std::mutex& io_mutex(DevCtx* c);
+int io_buffer(DevCtx* c, size_t bytes, char** out);
+bool is_device_ptr(const void* p);
+void phase_enable(int on);
+void phase_read(double* ms, long* calls, double* flops);
+void phase_reset();
+}  // namespace la
+
+using namespace la;
+
+namespace {
+
+bool lsame(char a, char b) { return (a | 0x20) == (b | 0x20); }
+bool is_trans(char t) { return lsame(t, 'T') || lsame(t, 'C'); }
+
+// ---- XERBLA state ----------------------------------------------------------
+std::mutex g_xmu;
+bool g_xrecord = false;
+char g_xname[16];
+int g_xinfo = 0;
+int g_xcalls = 0;
+
+void call_xerbla(const char* name, int info) {
+  const int p = -info;
+  xerbla_(name, &p, std::strlen(name));
+}
+
+int hipc(hipError_t e) {
+  if (e == hipSuccess) return 0;
+  std::fprintf(stderr, "lapack_amd: HIP error %d (%s)\n", (int)e, hipGetErrorString(e));
+  return LA_ERR_HIP;
+}
+#define CK(x)                     \
+  do {                            \
+    int _r = hipc(x);             \
+    if (_r) return _r;            \
+  } while (0)
+
+size_t al256(size_t x) { return (x + 255) / 256 * 256; }
+
+// One host-or-device array seen through a device pointer.  Host arrays are
+// copied into the staging buffer (tightly packed, ld = rows).
+template <class T>
+struct Arr {
+  T* user = nullptr;
+  long user_ld = 0;
+  int rows = 0, cols = 0;
+  T* dev = nullptr;
+  long ld = 0;
+  bool host = false;
+  size_t bytes() const { return host ? al256(sizeof(T) * (size_t)std::max(rows, 1) * std::max(cols, 1)) : 0; }
+};
+template <class T>
+Arr<T> arr(T* p, long ld, int rows, int cols) {
+  Arr<T> a;
+  a.user = p;
+  a.user_ld = ld;
+  a.rows = rows;
+  a.cols = cols;
+  a.host = (p != nullptr) && !is_device_ptr(p);
+  a.dev = p;
+  a.ld = ld;
+  return a;
+}
+template <class T>
+void place(Arr<T>& a, char*& cur) {
+  if (!a.host) return;
+  a.dev = reinterpret_cast<T*>(cur);
+  a.ld = std::max(a.rows, 1);
+  cur += a.bytes();
+}
+template <class T>
+int to_dev(Arr<T>& a, hipStream_t s) {
+  if (!a.host || a.rows <= 0 || a.cols <= 0) return 0;
+  CK(hipMemcpy2DAsync(a.dev, sizeof(T) * a.ld, a.user, sizeof(T) * a.user_ld, sizeof(T) * a.rows, a.cols,
+                      hipMemcpyHostToDevice, s));
+  return 0;
+}
+template <class T>
+int to_host(Arr<T>& a, hipStream_t s) {
+  if (!a.host || a.rows <= 0 || a.cols <= 0) return 0;
+  CK(hipMemcpy2DAsync(a.user, sizeof(T) * a.user_ld, a.dev, sizeof(T) * a.ld, sizeof(T) * a.rows, a.cols,
+                      hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+struct Call {
+  DevCtx* c = nullptr;
+  hipStream_t s = nullptr;
+  std::unique_lock<std::mutex> lk;
+  int open() {
+    c = ctx_for_current();
+    if (!c) return LA_ERR_NODEVICE;
+    s = ctx_stream(c);
+    lk = std::unique_lock<std::mutex>(io_mutex(c));
+    return 0;
+  }
+};
+
+// ---- generic synchronous drivers (host or device pointers) ----------------
+
+template <class T>
+int getrf_sync(int m, int n, T* a, int lda, int* ipiv) {
+  if (m < 0) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max(1, m)) return -4;
+  if (m == 0 || n == 0) return 0;
+  Call call;
+  int rc = call.open();
+  if (rc) return rc;
+  const int mn = std::min(m, n);
+  Arr<T> A = arr<T>(a, lda, m, n);
+  Arr<int> P = arr<int>(ipiv, mn, mn, 1);
+  size_t need = A.bytes() + P.bytes() + 256;
+  char* io = nullptr;
+  rc = io_buffer(call.c, need, &io);
+  if (rc) return rc;
+  char* cur = io;
+  int* dinfo = reinterpret_cast<int*>(cur);
+  cur += 256;
+  place(A, cur);
+  place(P, cur);
+  if ((rc = to_dev(A, call.s))) return rc;
+  rc = getrf_device<T>(m, n, A.dev, A.ld, P.dev, dinfo, call.s);
+  if (rc) return rc;
+  if ((rc = to_host(A, call.s))) return rc;
+  if ((rc = to_host(P, call.s))) return rc;
+  int info = 0;
+  CK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, call.s));
+  CK(hipStreamSynchronize(call.s));
+  return info;
+}
+
+template <class T>
+int getrs_check(char trans, int n, int nrhs, int lda, int ldb) {
+  if (!lsame(trans, 'N') && !is_trans(trans)) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (lda < std::max(1, n)) return -5;
+  if (ldb < std::max(1, n)) return -8;
+  return 0;
+}
+
+template <class T>
+int getrs_sync(char trans, int n, int nrhs, const T* a, int lda, const int* ipiv, T* b, int ldb) {
+  int rc = getrs_check<T>(trans, n, nrhs, lda, ldb);
+  if (rc) return rc;
+  if (n == 0 || nrhs == 0) return 0;
+  Call call;
+  if ((rc = call.open())) return rc;
+  Arr<T> A = arr<T>(const_cast<T*>(a), lda, n, n);
+  Arr<int> P = arr<int>(const_cast<int*>(ipiv), n, n, 1);
+  Arr<T> B = arr<T>(b, ldb, n, nrhs);
+  char* io = nullptr;
+  if ((rc = io_buffer(call.c, A.bytes() + P.bytes() + B.bytes() + 256, &io))) return rc;
+  char* cur = io;
+  place(A, cur);
+  place(P, cur);
+  place(B, cur);
+  if ((rc = to_dev(A, call.s)) || (rc = to_dev(P, call.s)) || (rc = to_dev(B, call.s))) return rc;
+  if ((rc = getrs_device<T>(trans, n, nrhs, A.dev, A.ld, P.dev, B.dev, B.ld, call.s))) return rc;
+  if ((rc = to_host(B, call.s))) return rc;
+  CK(hipStreamSynchronize(call.s));
+  return 0;
+}
+
+int gesv_sync(int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb) {
+  if (n < 0) return -1;
+  if (nrhs < 0) return -2;
+  if (lda < std::max(1, n)) return -4;
+  if (ldb < std::max(1, n)) return -7;
+  if (n == 0) return 0;
+  Call call;
+  int rc = call.open();
+  if (rc) return rc;
+  Arr<double> A = arr<double>(a, lda, n, n);
+  Arr<int> P = arr<int>(ipiv, n, n, 1);
+  Arr<double> B = arr<double>(b, ldb, n, nrhs);
+  char* io = nullptr;
+  if ((rc = io_buffer(call.c, A.bytes() + P.bytes() + B.bytes() + 256, &io))) return rc;
+  char* cur = io;
+  int* dinfo = reinterpret_cast<int*>(cur);
+  cur += 256;
+  place(A, cur);
+  place(P, cur);
+  place(B, cur);
+  if ((rc = to_dev(A, call.s)) || (rc = to_dev(B, call.s))) return rc;
+  if ((rc = getrf_device<double>(n, n, A.dev, A.ld, P.dev, dinfo, call.s))) return rc;
+  int info = 0;
+  CK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, call.s));
+  CK(hipStreamSynchronize(call.s));
+  // DGESV solves only when the factorization is nonsingular (SRC/dgesv.f:170-175).
+  if (info == 0 && nrhs > 0)
+    if ((rc = getrs_device<double>('N', n, nrhs, A.dev, A.ld, P.dev, B.dev, B.ld, call.s))) return rc;
+  if ((rc = to_host(A, call.s)) || (rc = to_host(P, call.s))) return rc;
+  if (info == 0 && (rc = to_host(B, call.s))) return rc;
+  CK(hipStreamSynchronize(call.s));
+  return info;
+}
+
+template <class T>
+int laswp_sync(int n, T* a, int lda, int k1, int k2, const int* ipiv, int incx) {
+  if (n <= 0 || incx == 0) return 0;
+  Call call;
+  int rc = call.open();
+  if (rc) return rc;
+  // Gather the IPIV entries the reference reads (IX0 + (I-I1)*INCX, SRC/dlaswp.f:138-150)
+  // into a contiguous host vector, and find how far down the rows go.
+  const bool dip = is_device_ptr(ipiv);
+  const int cnt = k2 - k1 + 1;
+  if (cnt <= 0) return 0;
+  const int ainc = incx > 0 ? incx : -incx;
+  const long span = (long)(cnt - 1) * ainc + 1;
+  std::vector<int> raw(span), piv(cnt);
+  if (dip) CK(hipMemcpy(raw.data(), ipiv + (k1 - 1), sizeof(int) * span, hipMemcpyDeviceToHost));
+  else std::memcpy(raw.data(), ipiv + (k1 - 1), sizeof(int) * span);
+  int maxrow = k2;
+  for (int i = 0; i < cnt; ++i) {
+    piv[i] = raw[(long)i * ainc];
+    maxrow = std::max(maxrow, piv[i]);
+  }
+  // In both directions the reference pairs row I with IPIV(K1+(I-K1)*|INCX|)
+  // (IX0 = K1+(K1-K2)*INCX for INCX < 0); only the order of the walk differs.
+  const int nrows = std::min(maxrow, lda);
+  Arr<T> A = arr<T>(a, lda, nrows, n);
+  char* io = nullptr;
+  if ((rc = io_buffer(call.c, A.bytes() + al256(sizeof(int) * (size_t)k2) + 256, &io))) return rc;
+  char* cur = io;
+  int* dpiv = reinterpret_cast<int*>(cur);
+  cur += al256(sizeof(int) * (size_t)k2);
+  place(A, cur);
+  // device IPIV laid out so that dpiv[k-1] is the pivot paired with row k
+  std::vector<int> full(k2, 0);
+  for (int i = 0; i < cnt; ++i) full[k1 - 1 + i] = piv[i];
+  CK(hipMemcpyAsync(dpiv, full.data(), sizeof(int) * (size_t)k2, hipMemcpyHostToDevice, call.s));
+  if ((rc = to_dev(A, call.s))) return rc;
+  if ((rc = laswp_device<T>(n, A.dev, A.ld, nrows, k1, k2, dpiv, incx > 0 ? 1 : -1, call.s))) return rc;
+  if ((rc = to_host(A, call.s))) return rc;
+  CK(hipStreamSynchronize(call.s));
+  return 0;
+}
+
+int gemm_check(char ta, char tb, int m, int n, int k, int lda, int ldb, int ldc) {
+  const bool nota = lsame(ta, 'N'), notb = lsame(tb, 'N');
+  const int nrowa = nota ? m : k, nrowb = notb ? k : n;
+  if (!nota && !is_trans(ta)) return 1;
+  if (!notb && !is_trans(tb)) return 2;
+  if (m < 0) return 3;
+  if (n < 0) return 4;
+  if (k < 0) return 5;
+  if (lda < std::max(1, nrowa)) return 8;
+  if (ldb < std::max(1, nrowb)) return 10;
+  if (ldc < std::max(1, m)) return 13;
+  return 0;
+}
+
+int gemm_sync(char ta, char tb, int m, int n, int k, double alpha, const double* a, int lda, const double* b,
+              int ldb, double beta, double* c, int ldc) {
+  int e = gemm_check(ta, tb, m, n, k, lda, ldb, ldc);
+  if (e) return -e;
+  if (m == 0 || n == 0) return 0;
+  Call call;
+  int rc = call.open();
+  if (rc) return rc;
+  const bool nota = lsame(ta, 'N'), notb = lsame(tb, 'N');
+  Arr<double> A = arr<double>(const_cast<double*>(a), lda, nota ? m : k, nota ? k : m);
+  Arr<double> B = arr<double>(const_cast<double*>(b), ldb, notb ? k : n, notb ? n : k);
+  Arr<double> C = arr<double>(c, ldc, m, n);
+  char* io = nullptr;
+  if ((rc = io_buffer(call.c, A.bytes() + B.bytes() + C.bytes() + 256, &io))) return rc;
+  char* cur = io;
+  place(A, cur);
+  place(B, cur);
+  place(C, cur);
+  if ((rc = to_dev(A, call.s)) || (rc = to_dev(B, call.s)) || (rc = to_dev(C, call.s))) return rc;
+  launch_dgemm(ta, tb, m, n, k, alpha, A.dev, A.ld, B.dev, B.ld, beta, C.dev, C.ld, call.s);
+  CK(hipGetLastError());
+  if ((rc = to_host(C, call.s))) return rc;
+  CK(hipStreamSynchronize(call.s));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- XERBLA -----------------------------------------------------------------
+// Reference behaviour (SRC/xerbla.f:86-88): print, then STOP.  Weak, so a
+// program's own XERBLA takes precedence at link time.
+__attribute__((weak)) void xerbla_(const char* srname, const int* info, size_t len) {
+  int n = (int)len;
+  while (n > 0 && srname[n - 1] == ' ') --n;
+  {
+    std::lock_guard<std::mutex> g(g_xmu);
+    const int k = std::min(n, (int)sizeof(g_xname) - 1);
+    std::memcpy(g_xname, srname, k);
+    g_xname[k] = 0;
+    g_xinfo = *info;
+    g_xcalls++;
+    if (g_xrecord) return;
+  }
+  std::printf(" ** On entry to %.*s parameter number %2d had an illegal value\n", n, srname, *info);
+  std::fflush(stdout);
+  std::exit(0);
+}
+
+void lapack_amd_xerbla_record(int on) {
+  std::lock_guard<std::mutex> g(g_xmu);
+  g_xrecord = on != 0;
+  g_xcalls = 0;
+  g_xinfo = 0;
+  g_xname[0] = 0;
+}
+int lapack_amd_xerbla_last(char* name_out, int cap) {
+  std::lock_guard<std::mutex> g(g_xmu);
+  if (name_out && cap > 0) {
+    std::strncpy(name_out, g_xname, (size_t)cap - 1);
+    name_out[cap - 1] = 0;
+  }
+  return g_xinfo;
+}
+int lapack_amd_xerbla_calls(void) {
+  std::lock_guard<std::mutex> g(g_xmu);
+  return g_xcalls;
+}
+
+// ---- Fortran drop-ins -----------------------------------------------------------
+void dgetrf_(const int* m, const int* n, double* a, const int* lda, int* ipiv, int* info) {
+  *info = getrf_sync<double>(*m, *n, a, *lda, ipiv);
+  if (*info < 0 && *info > -1000) call_xerbla("DGETRF", *info);
+}
+void dgetrf2_(const int* m, const int* n, double* a, const int* lda, int* ipiv, int* info) {
+  *info = getrf_sync<double>(*m, *n, a, *lda, ipiv);
+  if (*info < 0 && *info > -1000) call_xerbla("DGETRF2", *info);
+}
+void sgetrf_(const int* m, const int* n, float* a, const int* lda, int* ipiv, int* info) {
+  *info = getrf_sync<float>(*m, *n, a, *lda, ipiv);
+  if (*info < 0 && *info > -1000) call_xerbla("SGETRF", *info);
+}
+void dgetrs_(const char* trans, const int* n, const int* nrhs, const double* a, const int* lda, const int* ipiv,
+             double* b, const int* ldb, int* info, size_t) {
+  *info = getrs_sync<double>(*trans, *n, *nrhs, a, *lda, ipiv, b, *ldb);
+  if (*info < 0 && *info > -1000) call_xerbla("DGETRS", *info);
+}
+void sgetrs_(const char* trans, const int* n, const int* nrhs, const float* a, const int* lda, const int* ipiv,
+             float* b, const int* ldb, int* info, size_t) {
+  *info = getrs_sync<float>(*trans, *n, *nrhs, a, *lda, ipiv, b, *ldb);
+  if (*info < 0 && *info > -1000) call_xerbla("SGETRS", *info);
+}
+void dgesv_(const int* n, const int* nrhs, double* a, const int* lda, int* ipiv, double* b, const int* ldb,
+            int* info) {
+  *info = gesv_sync(*n, *nrhs, a, *lda, ipiv, b, *ldb);
+  if (*info < 0 && *info > -1000) call_xerbla("DGESV ", *info);
+}
+void dlaswp_(const int* n, double* a, const int* lda, const int* k1, const int* k2, const int* ipiv,
+             const int* incx) {
+  (void)laswp_sync<double>(*n, a, *lda, *k1, *k2, ipiv, *incx);
+}
+void dgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const double* alpha,
+            const double* a, const int* lda, const double* b, const int* ldb, const double* beta, double* c,
+            const int* ldc, size_t, size_t) {
+  const int rc = gemm_sync(*transa, *transb, *m, *n, *k, *alpha, a, *lda, b, *ldb, *beta, c, *ldc);
+  if (rc < 0 && rc > -1000) call_xerbla("DGEMM ", rc);
+}
+
+// ---- value-argument entry points ------------------------------------------
+int lapack_amd_dgetrf(int m, int n, double* a, int lda, int* ipiv) { return getrf_sync<double>(m, n, a, lda, ipiv); }
+int lapack_amd_sgetrf(int m, int n, float* a, int lda, int* ipiv) { return getrf_sync<float>(m, n, a, lda, ipiv); }
+int lapack_amd_dgetrs(char trans, int n, int nrhs, const double* a, int lda, const int* ipiv, double* b, int ldb) {
+  return getrs_sync<double>(trans, n, nrhs, a, lda, ipiv, b, ldb);
+}
+int lapack_amd_sgetrs(char trans, int n, int nrhs, const float* a, int lda, const int* ipiv, float* b, int ldb) {
+  return getrs_sync<float>(trans, n, nrhs, a, lda, ipiv, b, ldb);
+}
+int lapack_amd_dgesv(int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb) {
+  return gesv_sync(n, nrhs, a, lda, ipiv, b, ldb);
+}
+int lapack_amd_dlaswp(int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx) {
+  return laswp_sync<double>(n, a, lda, k1, k2, ipiv, incx);
+}
+int lapack_amd_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double* a, int lda, const double* b,
+                     int ldb, double beta, double* c, int ldc) {
+  return gemm_sync(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+// ---- device entry points ----------------------------------------------------
+int lapack_amd_dgetrf_async(int m, int n, double* d_a, int lda, int* d_ipiv, int* d_info, void* stream) {
+  if (m < 0) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max(1, m)) return -4;
+  return getrf_device<double>(m, n, d_a, lda, d_ipiv, d_info, (hipStream_t)stream);
+}
+int lapack_amd_sgetrf_async(int m, int n, float* d_a, int lda, int* d_ipiv, int* d_info, void* stream) {
+  if (m < 0) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max(1, m)) return -4;
+  return getrf_device<float>(m, n, d_a, lda, d_ipiv, d_info, (hipStream_t)stream);
+}
+int lapack_amd_dgetrs_async(char trans, int n, int nrhs, const double* d_a, int lda, const int* d_ipiv, double* d_b,
+                            int ldb, void* stream) {
+  int rc = getrs_check<double>(trans, n, nrhs, lda, ldb);
+  if (rc) return rc;
+  return getrs_device<double>(trans, n, nrhs, d_a, lda, d_ipiv, d_b, ldb, (hipStream_t)stream);
+}
+int lapack_amd_sgetrs_async(char trans, int n, int nrhs, const float* d_a, int lda, const int* d_ipiv, float* d_b,
+                            int ldb, void* stream) {
+  int rc = getrs_check<float>(trans, n, nrhs, lda, ldb);
+  if (rc) return rc;
+  return getrs_device<float>(trans, n, nrhs, d_a, lda, d_ipiv, d_b, ldb, (hipStream_t)stream);
+}
+int lapack_amd_dlaswp_async(int n, double* d_a, int lda, int k1, int k2, const int* d_ipiv, int incx, void* stream) {
+  if (incx != 1 && incx != -1) return LA_ERR_LIMIT;  // strided IPIV: use lapack_amd_dlaswp
+  return laswp_device<double>(n, d_a, lda, lda, k1, k2, d_ipiv, incx, (hipStream_t)stream);
+}
+int lapack_amd_dgemm_async(char ta, char tb, int m, int n, int k, double alpha, const double* d_a, int lda,
+                           const double* d_b, int ldb, double beta, double* d_c, int ldc, void* stream) {
+  int e = gemm_check(ta, tb, m, n, k, lda, ldb, ldc);
+  if (e) return -e;
+  launch_dgemm(ta, tb, m, n, k, alpha, d_a, lda, d_b, ldb, beta, d_c, ldc, (hipStream_t)stream);
+  return hipc(hipGetLastError());
+}
+int lapack_amd_sgemm_async(char ta, char tb, int m, int n, int k, float alpha, const float* d_a, int lda,
+                           const float* d_b, int ldb, float beta, float* d_c, int ldc, void* stream) {
+  int e = gemm_check(ta, tb, m, n, k, lda, ldb, ldc);
+  if (e) return -e;
+  launch_sgemm(ta, tb, m, n, k, alpha, d_a, lda, d_b, ldb, beta, d_c, ldc, (hipStream_t)stream);
+  return hipc(hipGetLastError());
+}
+
+// ---- tuning and tools --------------------------------------------------------
+void lapack_amd_set_tuning(int nb, int leaf) { set_tuning(nb, leaf); }
+void lapack_amd_get_tuning(int* nb, int* leaf) {
+  if (nb) *nb = tune_nb();
+  if (leaf) *leaf = tune_leaf();
+}
+void lapack_amd_phase_enable(int on) { phase_enable(on); }
+void lapack_amd_phase_reset(void) { phase_reset(); }
+void lapack_amd_phase_read(double* ms, long* calls, double* flops) { phase_read(ms, calls, flops); }
+const char* lapack_amd_version(void) { return "lapack_amd 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// la_common.h -- shared definitions for the gfx950 LU solve library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+typedef double la_d4 __attribute__((ext_vector_type(4)));
+typedef float la_f4 __attribute__((ext_vector_type(4)));
+
+// Library error codes (negative, below every LAPACK argument code).
+enum {
+  LA_ERR_HIP = -1001,       // a HIP runtime call failed
+  LA_ERR_NOMEM = -1002,     // device or pinned allocation failed
+  LA_ERR_SYNC = -1003,      // a cooperative panel kernel timed out on its grid barrier
+  LA_ERR_NODEVICE = -1004,  // no GPU visible
+  LA_ERR_LIMIT = -1005,     // a size beyond what this build supports
+};
+
+namespace la {
+
+// ---- device helpers -------------------------------------------------------
+
+// Device-coherent (agent scope) relaxed loads/stores: global_load/store with
+// sc1, visible across the 8 XCDs' L2s without an L2 write-back.
+template <class T> __device__ __forceinline__ void st_dev(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ T ld_dev(const T* p) {
+  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Wait until every vector-memory operation this wave issued has completed
+// (s_waitcnt vmcnt(0); gfx9 encoding with expcnt/lgkmcnt left at max).
+__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// ---- host-side launchers (defined in the .hip files) ---------------------
+
+// C <- alpha*op(A)*op(B) + beta*C, column-major, FP64 MFMA.
+void launch_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double* A, long lda,
+                  const double* B, long ldb, double beta, double* C, long ldc, hipStream_t s);
+// Same in FP32 (MFMA f32).
+void launch_sgemm(char ta, char tb, int m, int n, int k, float alpha, const float* A, long lda,
+                  const float* B, long ldb, float beta, float* C, long ldc, hipStream_t s);
+
+// out(j,i) = in(i,j): in is rows x cols with ld_in, out cols x rows with ld_out.
+void launch_transpose_d(int rows, int cols, const double* in, long ld_in, double* out, long ld_out,
+                        hipStream_t s);
+void launch_transpose_s(int rows, int cols, const float* in, long ld_in, float* out, long ld_out,
+                        hipStream_t s);
+// Cast/transposing variants used by the mixed-precision solver.
+void launch_transpose_d2s(int rows, int cols, const double* in, long ld_in, float* out, long ld_out,
+                          int* overflow, hipStream_t s);
+
+// Interchange columns r <-> ipiv[r]-1 (sequentially for r = k1..k2-1, 0-based
+// k's, 1-based ipiv) of T, restricted to rows [r0, r1).  In the transposed
+// workspace T = A^T this is DLASWP on A's columns r0..r1-1.
+void launch_laswp_cols_d(double* T, long ldt, int r0, int r1, int k1, int k2, const int* ipiv,
+                         hipStream_t s);
+void launch_laswp_cols_s(float* T, long ldt, int r0, int r1, int k1, int k2, const int* ipiv,
+                         hipStream_t s);
+// DLASWP on a column-major matrix: perm (nrows ints) <- the permutation of
+// the interchange sequence ipiv[k1..k2-1] (1-based entries, incx = +1/-1),
+// then rows are gathered through it.
+void launch_swaps_to_perm(int nrows, int k1, int k2, const int* ipiv, int incx, int* perm, hipStream_t s);
+void launch_gather_rows_d(int nrows, int ncols, const int* perm, const double* src, long lds, double* dst,
+                          long ldd, hipStream_t s);
+void launch_gather_rows_s(int nrows, int ncols, const int* perm, const float* src, long lds, float* dst,
+                          long ldd, hipStream_t s);
+
+// Batched triangular block inverses along the diagonal of an n x n matrix:
+// block d (rows/cols d*nb..) of tri(U) inverted into W(:, d*nb..), nb <= 256.
+void launch_trtri_blocks_d(bool upper, bool unit, int n, int nb, const double* U, long ldu, double* W,
+                           long ldw, hipStream_t s);
+void launch_trtri_blocks_s(bool upper, bool unit, int n, int nb, const float* U, long ldu, float* W,
+                           long ldw, hipStream_t s);
+// Inverse of the unit upper triangular nb x nb block U (ldu), into W (ldw).
+void launch_trtri_uu_d(int nb, const double* U, long ldu, double* W, long ldw, hipStream_t s);
+void launch_trtri_uu_s(int nb, const float* U, long ldu, float* W, long ldw, hipStream_t s);
+// Inverse of the non-unit upper triangular nb x nb block U, into W.
+void launch_trtri_un_d(int nb, const double* U, long ldu, double* W, long ldw, hipStream_t s);
+void launch_trtri_un_s(int nb, const float* U, long ldu, float* W, long ldw, hipStream_t s);
+
+// Copy a rows x cols column-major block.
+void launch_copy_d(int rows, int cols, const double* src, long lds, double* dst, long ldd, hipStream_t s);
+void launch_copy_s(int rows, int cols, const float* src, long lds, float* dst, long ldd, hipStream_t s);
+void launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s);
+void launch_fill_i32(int* p, size_t n, int v, hipStream_t s);
+// info <- LA_ERR_SYNC when a panel barrier timed out (err != 0).
+void launch_fold_err(const int* err, int* info, hipStream_t s);
+
+// Cooperative panel (leaf) factorization on the transposed workspace: see panel.hip.
+struct PanelSync {
+  unsigned* counters;  // one barrier counter per launch (pre-zeroed)
+  double* xchg;        // exchange area (candidates and rows), device memory
+  int* err;            // set to 1 when a barrier wait timed out
+};
+int panel_max_width(int h);  // widest leaf the panel kernel takes at height h
+size_t panel_xchg_bytes(int max_groups);
+void launch_getf2_panel_d(double* T, long ldt, int j0, int w, int m, int* ipiv, int* info,
+                          unsigned* counter, double* xchg, int* err, hipStream_t s);
+void launch_getf2_panel_s(float* T, long ldt, int j0, int w, int m, int* ipiv, int* info,
+                          unsigned* counter, double* xchg, int* err, hipStream_t s);
+
+}  // namespace la

@@ -1,0 +1,589 @@
+// lu_oracle.cpp -- CPU restatement of Reference-LAPACK's dense LU solve path.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+// calls this file; it is the checker that the GPU path is compared against
+// (tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py).
+//
+// Every routine follows the loop order of the reference routine it names so
+// that, compiled with `-O2 -ffp-contract=off` (no FMA contraction, no
+// -march=native), it is bit-identical to the Fortran reference built from
+// its own sources (oracle/Makefile -> oracle/_ref).  Parity of this file with
+// that build is asserted by tests/test_oracle.py on a size sweep.
+//
+// Index conventions: arrays are column-major with leading dimension ld; the
+// routines use Fortran's 1-based row/column numbers internally (AT(i,j)) and
+// keep IPIV 1-based exactly as the reference does.
+//
+// Pinning (how each function is checked, see tests/test_oracle.py):
+//   iamax, scal, laswp, trsm_left, gemm, getrf2, getrf, getrs  -- bitwise
+//     equality with the flang-built reference (oracle/_ref) on seeded inputs;
+//     LU also by the DGET01 ratio, x by DGET02, and brute force on tiny cases.
+//   lange, get01/get02/get08                                   -- closed forms
+//     on hand-built matrices (tests/test_oracle.py) and agreement with _ref.
+//   lag2s / lag2d / dsgesv                                     -- bitwise
+//     equality with _ref's DSGESV (ITER, IPIV, X).
+//   geequ / geequb / laqge                                     -- DCHKEQ
+//     closed form (TESTING/LIN/dchkeq.f:107-173) and equality with _ref.
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <cfloat>
+#include <vector>
+#include <algorithm>
+
+namespace {
+
+template <class T> struct Mach;
+// DLAMCH('E') = EPSILON(0d0)*0.5 (INSTALL/dlamch.f:101-107); 'S' = TINY since
+// 1/HUGE < TINY (INSTALL/dlamch.f:111-123); 'O' = HUGE.
+template <> struct Mach<double> {
+  static constexpr double eps = 0x1p-53;
+  static constexpr double sfmin = 0x1p-1022;
+  static constexpr double ovfl = DBL_MAX;
+};
+// SLAMCH, same definitions in single precision (INSTALL/slamch.f).
+template <> struct Mach<float> {
+  static constexpr float eps = 0x1p-24f;
+  static constexpr float sfmin = 0x1p-126f;
+  static constexpr float ovfl = FLT_MAX;
+};
+
+template <class T> inline T tabs(T x) { return x < T(0) ? -x : (x == T(0) ? T(0) : x); }
+
+// Column-major element (i,j), 1-based, leading dimension ld.
+#define AT(a, i, j, ld) (a)[(size_t)((i)-1) + (size_t)((j)-1) * (size_t)(ld)]
+
+// IDAMAX: first index of max |x|; strict '>' so ties keep the lowest index
+// and a NaN is only ever returned at position 1 (BLAS/SRC/idamax.f:93-122).
+template <class T> int iamax(int n, const T* x, int incx) {
+  if (n < 1 || incx <= 0) return 0;
+  if (n == 1) return 1;
+  int imax = 1;
+  T dmax = std::fabs(x[0]);
+  size_t ix = (size_t)incx;
+  for (int i = 2; i <= n; ++i, ix += (size_t)incx) {
+    T v = std::fabs(x[ix]);
+    if (v > dmax) { imax = i; dmax = v; }
+  }
+  return imax;
+}
+
+// DSCAL x <- da*x; quick return when da == 1 (BLAS/SRC/dscal.f:104-135).
+template <class T> void scal(int n, T da, T* x, int incx) {
+  if (n <= 0 || incx <= 0 || da == T(1)) return;
+  for (int i = 0; i < n; ++i) x[(size_t)i * incx] = da * x[(size_t)i * incx];
+}
+
+// DLASWP: sequential row interchanges row I <-> row IPIV(IX) for I = K1..K2
+// (INCX > 0) or K2..K1 (INCX < 0) (SRC/dlaswp.f:138-183).  The reference
+// sweeps 32-column blocks; the swaps of different columns are independent,
+// so applying the whole sequence column block by column block or row by row
+// moves the same values.
+template <class T> void laswp(int n, T* a, int lda, int k1, int k2, const int* ipiv, int incx) {
+  int ix0, i1, i2, inc;
+  if (incx > 0) { ix0 = k1; i1 = k1; i2 = k2; inc = 1; }
+  else if (incx < 0) { ix0 = k1 + (k1 - k2) * incx; i1 = k2; i2 = k1; inc = -1; }
+  else return;
+  int ix = ix0;
+  for (int i = i1; inc > 0 ? i <= i2 : i >= i2; i += inc) {
+    int ip = ipiv[ix - 1];
+    if (ip != i) {
+      for (int k = 1; k <= n; ++k) std::swap(AT(a, i, k, lda), AT(a, ip, k, lda));
+    }
+    ix += incx;
+  }
+}
+
+// DTRSM with SIDE='L' (BLAS/SRC/dtrsm.f:261-329): B <- alpha*op(A)^-1*B.
+template <class T>
+void trsm_left(char uplo, char trans, char diag, int m, int n, T alpha, const T* a, int lda,
+               T* b, int ldb) {
+  if (m == 0 || n == 0) return;
+  const bool upper = (uplo == 'U' || uplo == 'u');
+  const bool notran = (trans == 'N' || trans == 'n');
+  const bool nounit = (diag == 'N' || diag == 'n');
+  if (alpha == T(0)) {
+    for (int j = 1; j <= n; ++j)
+      for (int i = 1; i <= m; ++i) AT(b, i, j, ldb) = T(0);
+    return;
+  }
+  if (notran) {
+    if (upper) {  // dtrsm.f:277-288
+      for (int j = 1; j <= n; ++j) {
+        for (int i = 1; i <= m; ++i) AT(b, i, j, ldb) = alpha * AT(b, i, j, ldb);
+        for (int k = m; k >= 1; --k) {
+          if (nounit) AT(b, k, j, ldb) = AT(b, k, j, ldb) / AT(a, k, k, lda);
+          const T bkj = AT(b, k, j, ldb);
+          for (int i = 1; i <= k - 1; ++i) AT(b, i, j, ldb) = AT(b, i, j, ldb) - bkj * AT(a, i, k, lda);
+        }
+      }
+    } else {  // dtrsm.f:289-300
+      for (int j = 1; j <= n; ++j) {
+        for (int i = 1; i <= m; ++i) AT(b, i, j, ldb) = alpha * AT(b, i, j, ldb);
+        for (int k = 1; k <= m; ++k) {
+          if (nounit) AT(b, k, j, ldb) = AT(b, k, j, ldb) / AT(a, k, k, lda);
+          const T bkj = AT(b, k, j, ldb);
+          for (int i = k + 1; i <= m; ++i) AT(b, i, j, ldb) = AT(b, i, j, ldb) - bkj * AT(a, i, k, lda);
+        }
+      }
+    }
+  } else {
+    if (upper) {  // dtrsm.f:306-316
+      for (int j = 1; j <= n; ++j)
+        for (int i = 1; i <= m; ++i) {
+          T temp = alpha * AT(b, i, j, ldb);
+          for (int k = 1; k <= i - 1; ++k) temp = temp - AT(a, k, i, lda) * AT(b, k, j, ldb);
+          if (nounit) temp = temp / AT(a, i, i, lda);
+          AT(b, i, j, ldb) = temp;
+        }
+    } else {  // dtrsm.f:317-327
+      for (int j = 1; j <= n; ++j)
+        for (int i = m; i >= 1; --i) {
+          T temp = alpha * AT(b, i, j, ldb);
+          for (int k = i + 1; k <= m; ++k) temp = temp - AT(a, k, i, lda) * AT(b, k, j, ldb);
+          if (nounit) temp = temp / AT(a, i, i, lda);
+          AT(b, i, j, ldb) = temp;
+        }
+    }
+  }
+}
+
+// DGEMM C <- alpha*op(A)*op(B) + beta*C (BLAS/SRC/dgemm.f:298-402).
+template <class T>
+void gemm(char transa, char transb, int m, int n, int k, T alpha, const T* a, int lda, const T* b,
+          int ldb, T beta, T* c, int ldc) {
+  const bool nota = (transa == 'N' || transa == 'n');
+  const bool notb = (transb == 'N' || transb == 'n');
+  if (m == 0 || n == 0 || ((alpha == T(0) || k == 0) && beta == T(1))) return;
+  if (alpha == T(0)) {
+    for (int j = 1; j <= n; ++j)
+      for (int i = 1; i <= m; ++i)
+        AT(c, i, j, ldc) = (beta == T(0)) ? T(0) : beta * AT(c, i, j, ldc);
+    return;
+  }
+  if (notb) {
+    if (nota) {  // dgemm.f:327-343
+      for (int j = 1; j <= n; ++j) {
+        if (beta == T(0)) { for (int i = 1; i <= m; ++i) AT(c, i, j, ldc) = T(0); }
+        else if (beta != T(1)) { for (int i = 1; i <= m; ++i) AT(c, i, j, ldc) = beta * AT(c, i, j, ldc); }
+        for (int l = 1; l <= k; ++l) {
+          const T temp = alpha * AT(b, l, j, ldb);
+          T* cj = &AT(c, 1, j, ldc);
+          const T* al = &AT(a, 1, l, lda);
+          for (int i = 0; i < m; ++i) cj[i] = cj[i] + temp * al[i];
+        }
+      }
+    } else {  // dgemm.f:348-360
+      for (int j = 1; j <= n; ++j)
+        for (int i = 1; i <= m; ++i) {
+          T temp = T(0);
+          for (int l = 1; l <= k; ++l) temp = temp + AT(a, l, i, lda) * AT(b, l, j, ldb);
+          AT(c, i, j, ldc) = (beta == T(0)) ? alpha * temp : alpha * temp + beta * AT(c, i, j, ldc);
+        }
+    }
+  } else {
+    if (nota) {  // dgemm.f:367-383
+      for (int j = 1; j <= n; ++j) {
+        if (beta == T(0)) { for (int i = 1; i <= m; ++i) AT(c, i, j, ldc) = T(0); }
+        else if (beta != T(1)) { for (int i = 1; i <= m; ++i) AT(c, i, j, ldc) = beta * AT(c, i, j, ldc); }
+        for (int l = 1; l <= k; ++l) {
+          const T temp = alpha * AT(b, j, l, ldb);
+          T* cj = &AT(c, 1, j, ldc);
+          const T* al = &AT(a, 1, l, lda);
+          for (int i = 0; i < m; ++i) cj[i] = cj[i] + temp * al[i];
+        }
+      }
+    } else {  // dgemm.f:388-400
+      for (int j = 1; j <= n; ++j)
+        for (int i = 1; i <= m; ++i) {
+          T temp = T(0);
+          for (int l = 1; l <= k; ++l) temp = temp + AT(a, l, i, lda) * AT(b, j, l, ldb);
+          AT(c, i, j, ldc) = (beta == T(0)) ? alpha * temp : alpha * temp + beta * AT(c, i, j, ldc);
+        }
+    }
+  }
+}
+
+// DGETRF2: recursive panel LU (SRC/dgetrf2.f:152-265).
+template <class T> int getrf2(int m, int n, T* a, int lda, int* ipiv) {
+  int info = 0;
+  if (m == 0 || n == 0) return 0;
+  if (m == 1) {  // dgetrf2.f:170-177
+    ipiv[0] = 1;
+    if (AT(a, 1, 1, lda) == T(0)) info = 1;
+  } else if (n == 1) {  // dgetrf2.f:179-214
+    const T sfmin = Mach<T>::sfmin;
+    const int i = iamax(m, a, 1);
+    ipiv[0] = i;
+    if (AT(a, i, 1, lda) != T(0)) {
+      if (i != 1) std::swap(AT(a, 1, 1, lda), AT(a, i, 1, lda));
+      if (std::fabs(AT(a, 1, 1, lda)) >= sfmin) {
+        scal(m - 1, T(1) / AT(a, 1, 1, lda), &AT(a, 2, 1, lda), 1);
+      } else {
+        for (int r = 1; r <= m - 1; ++r) AT(a, 1 + r, 1, lda) = AT(a, 1 + r, 1, lda) / AT(a, 1, 1, lda);
+      }
+    } else {
+      info = 1;
+    }
+  } else {  // dgetrf2.f:220-263
+    const int n1 = std::min(m, n) / 2;
+    const int n2 = n - n1;
+    int iinfo = getrf2(m, n1, a, lda, ipiv);
+    if (info == 0 && iinfo > 0) info = iinfo;
+    laswp(n2, &AT(a, 1, n1 + 1, lda), lda, 1, n1, ipiv, 1);
+    trsm_left('L', 'N', 'U', n1, n2, T(1), a, lda, &AT(a, 1, n1 + 1, lda), lda);
+    gemm('N', 'N', m - n1, n2, n1, T(-1), &AT(a, n1 + 1, 1, lda), lda, &AT(a, 1, n1 + 1, lda), lda,
+         T(1), &AT(a, n1 + 1, n1 + 1, lda), lda);
+    iinfo = getrf2(m - n1, n2, &AT(a, n1 + 1, n1 + 1, lda), lda, ipiv + n1);
+    if (info == 0 && iinfo > 0) info = iinfo + n1;
+    for (int i = n1 + 1; i <= std::min(m, n); ++i) ipiv[i - 1] += n1;
+    laswp(n1, a, lda, n1 + 1, std::min(m, n), ipiv, 1);
+  }
+  return info;
+}
+
+// DGETRF: blocked right-looking LU (SRC/dgetrf.f:144-220); nb plays the role
+// of ILAENV(1,'DGETRF',...) which is 64 in the reference (SRC/ilaenv.f:289-295).
+template <class T> int getrf(int m, int n, T* a, int lda, int* ipiv, int nb) {
+  if (m < 0) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max(1, m)) return -4;
+  if (m == 0 || n == 0) return 0;
+  int info = 0;
+  if (nb <= 1 || nb >= std::min(m, n)) return getrf2(m, n, a, lda, ipiv);
+  for (int j = 1; j <= std::min(m, n); j += nb) {
+    const int jb = std::min(std::min(m, n) - j + 1, nb);
+    const int iinfo = getrf2(m - j + 1, jb, &AT(a, j, j, lda), lda, ipiv + (j - 1));
+    if (info == 0 && iinfo > 0) info = iinfo + j - 1;
+    for (int i = j; i <= std::min(m, j + jb - 1); ++i) ipiv[i - 1] = j - 1 + ipiv[i - 1];
+    laswp(j - 1, a, lda, j, j + jb - 1, ipiv, 1);
+    if (j + jb <= n) {
+      laswp(n - j - jb + 1, &AT(a, 1, j + jb, lda), lda, j, j + jb - 1, ipiv, 1);
+      trsm_left('L', 'N', 'U', jb, n - j - jb + 1, T(1), &AT(a, j, j, lda), lda, &AT(a, j, j + jb, lda), lda);
+      if (j + jb <= m)
+        gemm('N', 'N', m - j - jb + 1, n - j - jb + 1, jb, T(-1), &AT(a, j + jb, j, lda), lda,
+             &AT(a, j, j + jb, lda), lda, T(1), &AT(a, j + jb, j + jb, lda), lda);
+    }
+  }
+  return info;
+}
+
+// DGETRS (SRC/dgetrs.f:157-218).
+template <class T>
+int getrs(char trans, int n, int nrhs, const T* a, int lda, const int* ipiv, T* b, int ldb) {
+  const bool notran = (trans == 'N' || trans == 'n');
+  if (!notran && !(trans == 'T' || trans == 't' || trans == 'C' || trans == 'c')) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (lda < std::max(1, n)) return -5;
+  if (ldb < std::max(1, n)) return -8;
+  if (n == 0 || nrhs == 0) return 0;
+  if (notran) {
+    laswp(nrhs, b, ldb, 1, n, ipiv, 1);
+    trsm_left('L', 'N', 'U', n, nrhs, T(1), a, lda, b, ldb);
+    trsm_left('U', 'N', 'N', n, nrhs, T(1), a, lda, b, ldb);
+  } else {
+    trsm_left('U', 'T', 'N', n, nrhs, T(1), a, lda, b, ldb);
+    trsm_left('L', 'T', 'U', n, nrhs, T(1), a, lda, b, ldb);
+    laswp(nrhs, b, ldb, 1, n, ipiv, -1);
+  }
+  return 0;
+}
+
+// DLANGE for NORM in {'M','1'/'O','I'} (SRC/dlange.f:115-185).
+template <class T> T lange(char norm, int m, int n, const T* a, int lda) {
+  if (std::min(m, n) == 0) return T(0);
+  T value = T(0);
+  if (norm == 'M' || norm == 'm') {
+    for (int j = 1; j <= n; ++j)
+      for (int i = 1; i <= m; ++i) {
+        T temp = std::fabs(AT(a, i, j, lda));
+        if (value < temp || std::isnan(temp)) value = temp;
+      }
+  } else if (norm == 'O' || norm == 'o' || norm == '1') {
+    for (int j = 1; j <= n; ++j) {
+      T sum = T(0);
+      for (int i = 1; i <= m; ++i) sum = sum + std::fabs(AT(a, i, j, lda));
+      if (value < sum || std::isnan(sum)) value = sum;
+    }
+  } else if (norm == 'I' || norm == 'i') {
+    std::vector<T> work((size_t)m, T(0));
+    for (int j = 1; j <= n; ++j)
+      for (int i = 1; i <= m; ++i) work[i - 1] = work[i - 1] + std::fabs(AT(a, i, j, lda));
+    for (int i = 0; i < m; ++i) {
+      T temp = work[i];
+      if (value < temp || std::isnan(temp)) value = temp;
+    }
+  }
+  return value;
+}
+
+// DLAG2S: FP64 -> FP32 by round to nearest; INFO=1 on |a| > SLAMCH('O')
+// (SRC/dlag2s.f:135-146).
+int lag2s(int m, int n, const double* a, int lda, float* sa, int ldsa) {
+  const double rmax = (double)FLT_MAX;
+  for (int j = 1; j <= n; ++j)
+    for (int i = 1; i <= m; ++i) {
+      const double v = AT(a, i, j, lda);
+      if (v < -rmax || v > rmax) return 1;
+      AT(sa, i, j, ldsa) = (float)v;
+    }
+  return 0;
+}
+
+// SLAG2D: FP32 -> FP64, exact (SRC/slag2d.f:122-128).
+void lag2d(int m, int n, const float* sa, int ldsa, double* a, int lda) {
+  for (int j = 1; j <= n; ++j)
+    for (int i = 1; i <= m; ++i) AT(a, i, j, lda) = (double)AT(sa, i, j, ldsa);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- FP64 path -----------------------------------------------------------
+int oracle_idamax(int n, const double* x, int incx) { return iamax(n, x, incx); }
+int oracle_isamax(int n, const float* x, int incx) { return iamax(n, x, incx); }
+void oracle_dscal(int n, double da, double* x, int incx) { scal(n, da, x, incx); }
+void oracle_dlaswp(int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx) {
+  laswp(n, a, lda, k1, k2, ipiv, incx);
+}
+void oracle_dtrsm_left(char uplo, char trans, char diag, int m, int n, double alpha, const double* a,
+                       int lda, double* b, int ldb) {
+  trsm_left(uplo, trans, diag, m, n, alpha, a, lda, b, ldb);
+}
+void oracle_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double* a, int lda,
+                  const double* b, int ldb, double beta, double* c, int ldc) {
+  gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+int oracle_dgetrf2(int m, int n, double* a, int lda, int* ipiv) { return getrf2(m, n, a, lda, ipiv); }
+int oracle_dgetrf(int m, int n, double* a, int lda, int* ipiv, int nb) { return getrf(m, n, a, lda, ipiv, nb); }
+int oracle_dgetrs(char trans, int n, int nrhs, const double* a, int lda, const int* ipiv, double* b, int ldb) {
+  return getrs(trans, n, nrhs, a, lda, ipiv, b, ldb);
+}
+double oracle_dlange(char norm, int m, int n, const double* a, int lda) { return lange(norm, m, n, a, lda); }
+
+// ---- FP32 twins (SRC/sgetrf.f, sgetrf2.f, sgetrs.f, slaswp.f, BLAS sgemm/strsm) ----
+void oracle_slaswp(int n, float* a, int lda, int k1, int k2, const int* ipiv, int incx) {
+  laswp(n, a, lda, k1, k2, ipiv, incx);
+}
+void oracle_sgemm(char ta, char tb, int m, int n, int k, float alpha, const float* a, int lda,
+                  const float* b, int ldb, float beta, float* c, int ldc) {
+  gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+void oracle_strsm_left(char uplo, char trans, char diag, int m, int n, float alpha, const float* a,
+                       int lda, float* b, int ldb) {
+  trsm_left(uplo, trans, diag, m, n, alpha, a, lda, b, ldb);
+}
+int oracle_sgetrf(int m, int n, float* a, int lda, int* ipiv, int nb) { return getrf(m, n, a, lda, ipiv, nb); }
+int oracle_sgetrs(char trans, int n, int nrhs, const float* a, int lda, const int* ipiv, float* b, int ldb) {
+  return getrs(trans, n, nrhs, a, lda, ipiv, b, ldb);
+}
+int oracle_dlag2s(int m, int n, const double* a, int lda, float* sa, int ldsa) { return lag2s(m, n, a, lda, sa, ldsa); }
+void oracle_slag2d(int m, int n, const float* sa, int ldsa, double* a, int lda) { lag2d(m, n, sa, ldsa, a, lda); }
+
+// DSGESV: FP32 LU + FP64 iterative refinement, FP64 fallback
+// (SRC/dsgesv.f:259-427).  work is n*nrhs doubles, swork n*(n+nrhs) floats.
+// Returns INFO; *iter receives ITER.  nb is the ILAENV block size (64).
+int oracle_dsgesv(int n, int nrhs, double* a, int lda, int* ipiv, const double* b, int ldb, double* x,
+                  int ldx, double* work, float* swork, int* iter, int nb) {
+  *iter = 0;
+  if (n < 0) return -1;
+  if (nrhs < 0) return -2;
+  if (lda < std::max(1, n)) return -4;
+  if (ldb < std::max(1, n)) return -7;
+  if (ldx < std::max(1, n)) return -9;
+  if (n == 0) return 0;
+  const int itermax = 30;
+  const double anrm = lange('I', n, n, a, lda);
+  const double eps = Mach<double>::eps;
+  const double cte = anrm * eps * std::sqrt((double)n) * 1.0;
+  float* sa = swork;
+  float* sx = swork + (size_t)n * n;
+  int info = 0;
+  auto residual = [&]() {  // WORK = B - A*X  (dsgesv.f:328-332)
+    for (int j = 1; j <= nrhs; ++j)
+      for (int i = 1; i <= n; ++i) AT(work, i, j, n) = AT(b, i, j, ldb);
+    gemm('N', 'N', n, nrhs, n, -1.0, a, lda, x, ldx, 1.0, work, n);
+  };
+  auto converged = [&]() {  // dsgesv.f:337-342
+    for (int i = 1; i <= nrhs; ++i) {
+      const double xnrm = std::fabs(AT(x, iamax(n, &AT(x, 1, i, ldx), 1), i, ldx));
+      const double rnrm = std::fabs(AT(work, iamax(n, &AT(work, 1, i, n), 1), i, n));
+      if (rnrm > xnrm * cte) return false;
+    }
+    return true;
+  };
+  bool fallback = false;
+  if (lag2s(n, nrhs, b, ldb, sx, n) != 0) { *iter = -2; fallback = true; }
+  if (!fallback && lag2s(n, n, a, lda, sa, n) != 0) { *iter = -2; fallback = true; }
+  if (!fallback && getrf(n, n, sa, n, ipiv, nb) != 0) { *iter = -3; fallback = true; }
+  if (!fallback) {
+    getrs('N', n, nrhs, sa, n, ipiv, sx, n);
+    lag2d(n, nrhs, sx, n, x, ldx);
+    residual();
+    if (converged()) { *iter = 0; return 0; }
+    for (int iiter = 1; iiter <= itermax; ++iiter) {
+      if (lag2s(n, nrhs, work, n, sx, n) != 0) { *iter = -2; fallback = true; break; }
+      getrs('N', n, nrhs, sa, n, ipiv, sx, n);
+      lag2d(n, nrhs, sx, n, work, n);
+      for (int i = 1; i <= nrhs; ++i)
+        for (int r = 1; r <= n; ++r) AT(x, r, i, ldx) = AT(x, r, i, ldx) + 1.0 * AT(work, r, i, n);
+      residual();
+      if (converged()) { *iter = iiter; return 0; }
+    }
+    if (!fallback) *iter = -itermax - 1;
+  }
+  // FP64 fallback (dsgesv.f:413-427)
+  info = getrf(n, n, a, lda, ipiv, nb);
+  if (info != 0) return info;
+  for (int j = 1; j <= nrhs; ++j)
+    for (int i = 1; i <= n; ++i) AT(x, i, j, ldx) = AT(b, i, j, ldb);
+  return getrs('N', n, nrhs, a, lda, ipiv, x, ldx);
+}
+
+// ---- equilibration (SRC/dgeequ.f:175-296, dgeequb.f:182-313, dlaqge.f:173-226) ----
+int oracle_dgeequ(int m, int n, const double* a, int lda, double* r, double* c, double* rowcnd,
+                  double* colcnd, double* amax, int radix_round) {
+  if (m < 0) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max(1, m)) return -4;
+  if (m == 0 || n == 0) { *rowcnd = 1; *colcnd = 1; *amax = 0; return 0; }
+  const double smlnum = Mach<double>::sfmin;
+  const double bignum = 1.0 / smlnum;
+  const double radix = 2.0;
+  const double logrdx = std::log(radix);
+  for (int i = 0; i < m; ++i) r[i] = 0;
+  for (int j = 1; j <= n; ++j)
+    for (int i = 1; i <= m; ++i) r[i - 1] = std::max(r[i - 1], std::fabs(AT(a, i, j, lda)));
+  if (radix_round)
+    for (int i = 0; i < m; ++i)
+      if (r[i] > 0) r[i] = std::pow(radix, (int)(std::log(r[i]) / logrdx));
+  double rcmin = bignum, rcmax = 0;
+  for (int i = 0; i < m; ++i) { rcmax = std::max(rcmax, r[i]); rcmin = std::min(rcmin, r[i]); }
+  *amax = rcmax;
+  if (rcmin == 0) {
+    for (int i = 0; i < m; ++i) if (r[i] == 0) return i + 1;
+  } else {
+    for (int i = 0; i < m; ++i) r[i] = 1.0 / std::min(std::max(r[i], smlnum), bignum);
+    *rowcnd = std::max(rcmin, smlnum) / std::min(rcmax, bignum);
+  }
+  for (int j = 0; j < n; ++j) c[j] = 0;
+  for (int j = 1; j <= n; ++j) {
+    for (int i = 1; i <= m; ++i) c[j - 1] = std::max(c[j - 1], std::fabs(AT(a, i, j, lda)) * r[i - 1]);
+    if (radix_round && c[j - 1] > 0) c[j - 1] = std::pow(radix, (int)(std::log(c[j - 1]) / logrdx));
+  }
+  rcmin = bignum; rcmax = 0;
+  for (int j = 0; j < n; ++j) { rcmin = std::min(rcmin, c[j]); rcmax = std::max(rcmax, c[j]); }
+  if (rcmin == 0) {
+    for (int j = 0; j < n; ++j) if (c[j] == 0) return m + j + 1;
+  } else {
+    for (int j = 0; j < n; ++j) c[j] = 1.0 / std::min(std::max(c[j], smlnum), bignum);
+    *colcnd = std::max(rcmin, smlnum) / std::min(rcmax, bignum);
+  }
+  return 0;
+}
+
+// Returns EQUED as a character ('N','R','C','B').
+char oracle_dlaqge(int m, int n, double* a, int lda, const double* r, const double* c, double rowcnd,
+                   double colcnd, double amax) {
+  if (m <= 0 || n <= 0) return 'N';
+  const double thresh = 0.1;
+  const double small = Mach<double>::sfmin / (Mach<double>::eps * 2.0);  // DLAMCH('P') = eps*base
+  const double large = 1.0 / small;
+  if (rowcnd >= thresh && amax >= small && amax <= large) {
+    if (colcnd >= thresh) return 'N';
+    for (int j = 1; j <= n; ++j) {
+      const double cj = c[j - 1];
+      for (int i = 1; i <= m; ++i) AT(a, i, j, lda) = cj * AT(a, i, j, lda);
+    }
+    return 'C';
+  } else if (colcnd >= thresh) {
+    for (int j = 1; j <= n; ++j)
+      for (int i = 1; i <= m; ++i) AT(a, i, j, lda) = r[i - 1] * AT(a, i, j, lda);
+    return 'R';
+  }
+  for (int j = 1; j <= n; ++j) {
+    const double cj = c[j - 1];
+    for (int i = 1; i <= m; ++i) AT(a, i, j, lda) = cj * r[i - 1] * AT(a, i, j, lda);
+  }
+  return 'B';
+}
+
+// ---- test ratios (TESTING/LIN) ----------------------------------------------
+// DGET01: ||L*U - P*A||_1 / (n * ||A||_1 * eps) (TESTING/LIN/dget01.f:144-207).
+// The product L*U is formed with plain loops, then the row interchanges are
+// undone with DLASWP(..., -1) as the reference does.
+double oracle_dget01(int m, int n, const double* a, int lda, const double* afac, int ldafac,
+                     const int* ipiv) {
+  if (m <= 0 || n <= 0) return 0.0;
+  const double eps = Mach<double>::eps;
+  const double anorm = lange('1', m, n, a, lda);
+  const int k = std::min(m, n);
+  std::vector<double> lu((size_t)m * n, 0.0);
+  // (L*U)(i,j) = sum_{p<=min(i,j,k)} L(i,p)*U(p,j), L unit lower (m x k), U upper (k x n).
+  for (int j = 1; j <= n; ++j) {
+    for (int p = 1; p <= std::min(j, k); ++p) {
+      const double upj = AT(afac, p, j, ldafac);
+      lu[(size_t)(p - 1) + (size_t)(j - 1) * m] += upj;  // L(p,p) = 1
+      for (int i = p + 1; i <= m; ++i) lu[(size_t)(i - 1) + (size_t)(j - 1) * m] += AT(afac, i, p, ldafac) * upj;
+    }
+  }
+  laswp(n, lu.data(), m, 1, k, ipiv, -1);
+  for (int j = 1; j <= n; ++j)
+    for (int i = 1; i <= m; ++i) lu[(size_t)(i - 1) + (size_t)(j - 1) * m] -= AT(a, i, j, lda);
+  double resid = lange('1', m, n, lu.data(), m);
+  if (anorm <= 0) return resid != 0 ? 1.0 / eps : 0.0;
+  return ((resid / (double)n) / anorm) / eps;
+}
+
+// DGET02 ('N'/'T'): max_j ||b_j - op(A) x_j||_1 / (||A|| ||x_j||_1 eps), no
+// factor n (TESTING/LIN/dget02.f:181-219).  b is not modified here.
+double oracle_dget02(char trans, int m, int n, int nrhs, const double* a, int lda, const double* x, int ldx,
+                     const double* b, int ldb) {
+  if (m <= 0 || n <= 0 || nrhs == 0) return 0.0;
+  const bool tr = (trans == 'T' || trans == 't' || trans == 'C' || trans == 'c');
+  const int n1 = tr ? n : m, n2 = tr ? m : n;
+  const double eps = Mach<double>::eps;
+  const double anorm = tr ? lange('I', m, n, a, lda) : lange('1', m, n, a, lda);
+  if (anorm <= 0) return 1.0 / eps;
+  std::vector<double> r((size_t)n1 * nrhs);
+  for (int j = 1; j <= nrhs; ++j)
+    for (int i = 1; i <= n1; ++i) r[(size_t)(i - 1) + (size_t)(j - 1) * n1] = AT(b, i, j, ldb);
+  gemm(tr ? 'T' : 'N', 'N', n1, nrhs, n2, -1.0, a, lda, x, ldx, 1.0, r.data(), n1);
+  double resid = 0;
+  for (int j = 1; j <= nrhs; ++j) {
+    double bnorm = 0, xnorm = 0;
+    for (int i = 1; i <= n1; ++i) bnorm += std::fabs(r[(size_t)(i - 1) + (size_t)(j - 1) * n1]);
+    for (int i = 1; i <= n2; ++i) xnorm += std::fabs(AT(x, i, j, ldx));
+    if (xnorm <= 0) resid = 1.0 / eps;
+    else resid = std::max(resid, ((bnorm / anorm) / xnorm) / eps);
+  }
+  return resid;
+}
+
+// DGET08: the infinity-norm form of DGET02 (TESTING/LIN/dget08.f:178-213).
+double oracle_dget08(char trans, int m, int n, int nrhs, const double* a, int lda, const double* x, int ldx,
+                     const double* b, int ldb) {
+  if (m <= 0 || n <= 0 || nrhs == 0) return 0.0;
+  const bool tr = (trans == 'T' || trans == 't' || trans == 'C' || trans == 'c');
+  const int n1 = tr ? n : m, n2 = tr ? m : n;
+  const double eps = Mach<double>::eps;
+  const double anorm = lange('I', n1, n2, a, lda);
+  if (anorm <= 0) return 1.0 / eps;
+  std::vector<double> r((size_t)n1 * nrhs);
+  for (int j = 1; j <= nrhs; ++j)
+    for (int i = 1; i <= n1; ++i) r[(size_t)(i - 1) + (size_t)(j - 1) * n1] = AT(b, i, j, ldb);
+  gemm(tr ? 'T' : 'N', 'N', n1, nrhs, n2, -1.0, a, lda, x, ldx, 1.0, r.data(), n1);
+  double resid = 0;
+  for (int j = 1; j <= nrhs; ++j) {
+    const double bnorm = std::fabs(r[(size_t)(iamax(n1, &r[(size_t)(j - 1) * n1], 1) - 1) + (size_t)(j - 1) * n1]);
+    const double xnorm = std::fabs(AT(x, iamax(n2, &AT(x, 1, j, ldx), 1), j, ldx));
+    if (xnorm <= 0) resid = 1.0 / eps;
+    else resid = std::max(resid, ((bnorm / anorm) / xnorm) / eps);
+  }
+  return resid;
+}
+
+}  // extern "C"

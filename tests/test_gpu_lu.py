@@ -1,0 +1,238 @@
+"""GPU parity of the plain FP64/FP32 LU path against the oracle.
+
+Bars (the reference's own test ratios):
+  * IPIV identical to the reference DGETRF (integer output).  A divergence is
+    accepted only at a documented near-tie: at the first differing step k the
+    reference's second-best candidate has |L(i,k)| >= 1 - 1e-9.
+  * DGET01 ||P*L*U - A||_1/(n ||A||_1 eps) < 30 (TESTING/LIN/dget01.f, threshold
+    TESTING/dtest.in:13), DGET02 ||B - A X||_1/(||A||_1 ||X||_1 eps) < 30.
+  * GEMM: DBLAT3's componentwise ratio |ct - cc|/(eps*g) < 16
+    (BLAS/TESTING/dblat3.f:2940-2960).
+  * INFO equal (exact zero pivots).
+"""
+import numpy as np
+import pytest
+
+import lapack_amd as la
+import oracle
+
+from gpu_util import from_dev, ivec, to_dev
+
+pytestmark = pytest.mark.gpu
+
+THRESH = 30.0
+
+
+def first_mismatch(p, q):
+    d = np.nonzero(np.asarray(p) != np.asarray(q))[0]
+    return int(d[0]) if len(d) else -1
+
+
+def near_tie(lu_ref, k):
+    """1 - second largest |L(i,k)|, i > k, of the reference factors: the gap
+    between the chosen pivot and the runner-up at step k."""
+    col = np.abs(lu_ref[k + 1:, k])
+    if len(col) == 0:
+        return 1.0
+    return 1.0 - col.max()
+
+
+def assert_ipiv(ipiv_gpu, ipiv_ref, lu_ref):
+    k = first_mismatch(ipiv_gpu, ipiv_ref)
+    if k >= 0:
+        gap = near_tie(lu_ref, k)
+        assert gap < 1e-9, f"IPIV differs at step {k} without a near-tie (gap {gap:.3e})"
+
+
+def gpu_getrf(a, dtype=np.float64):
+    m, n = a.shape
+    d_a = to_dev(np.asarray(a, dtype=dtype))
+    d_p = ivec(min(m, n))
+    fn = la.dgetrf if dtype == np.float64 else la.sgetrf
+    info = fn(m, n, d_a, max(1, m), d_p)
+    return from_dev(d_a, (m, n)), d_p.cpu().numpy()[:min(m, n)], info
+
+
+def dmmch_ratio(ta, tb, alpha, a, b, beta, c, cc, eps):
+    opa = a.T if ta == "T" else a
+    opb = b.T if tb == "T" else b
+    ct = oracle.dgemm(ta, tb, alpha, a, b, beta, c)
+    g = abs(alpha) * (np.abs(opa) @ np.abs(opb)) + abs(beta) * np.abs(c)
+    err = np.abs(ct - cc) / eps
+    nz = g != 0
+    err[nz] /= g[nz]
+    return err.max() if err.size else 0.0
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (128, 128, 16), (129, 130, 131), (300, 70, 257)])
+def test_dgemm_dblat3_ratio(dev, ta, tb, m, n, k):
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    a = np.asfortranarray(rng.uniform(-1, 1, (m, k) if ta == "N" else (k, m)))
+    b = np.asfortranarray(rng.uniform(-1, 1, (k, n) if tb == "N" else (n, k)))
+    c = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    for alpha, beta in [(1.0, 0.0), (-1.0, 1.0), (0.7, -1.3)]:
+        d_a, d_b, d_c = to_dev(a), to_dev(b), to_dev(c)
+        rc = la.dgemm(ta, tb, m, n, k, alpha, d_a, a.shape[0], d_b, b.shape[0], beta, d_c, m)
+        assert rc == 0
+        cc = from_dev(d_c, (m, n))
+        assert dmmch_ratio(ta, tb, alpha, a, b, beta, c, cc, 2.0 ** -52) < 16.0
+
+
+def test_dgemm_beta_zero_does_not_read_c(dev):
+    m, n, k = 70, 40, 30
+    rng = np.random.default_rng(1)
+    a = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+    b = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    c = np.full((m, n), np.nan, order="F")
+    d_c = to_dev(c)
+    assert la.dgemm("N", "N", m, n, k, 1.0, to_dev(a), m, to_dev(b), k, 0.0, d_c, m) == 0
+    cc = from_dev(d_c, (m, n))
+    assert np.isfinite(cc).all()
+    np.testing.assert_allclose(cc, a @ b, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 10, 50, 63, 64, 65, 127, 128, 129, 255, 256, 257, 300, 513, 1024])
+def test_dgetrf_square_parity(dev, n):
+    a = la.matrix(2, 1988 + n, n, n)
+    lu_r, p_r, info_r = oracle.dgetrf(a)
+    lu, p, info = gpu_getrf(a)
+    assert info == info_r == 0
+    assert_ipiv(p, p_r, lu_r)
+    assert oracle.dget01(a, lu, p) < THRESH
+
+
+@pytest.mark.parametrize("m,n", [(1, 5), (5, 1), (70, 30), (30, 70), (300, 257), (257, 300), (1000, 64)])
+def test_dgetrf_rect_parity(dev, m, n):
+    a = la.matrix(3, 1991 + m + n, m, n)
+    lu_r, p_r, info_r = oracle.dgetrf(a)
+    lu, p, info = gpu_getrf(a)
+    assert info == info_r
+    assert_ipiv(p, p_r, lu_r)
+    assert oracle.dget01(a, lu, p) < THRESH
+
+
+@pytest.mark.parametrize("nb,leaf", [(1, 1), (3, 3), (20, 8), (64, 64), (128, 32), (256, 64)])
+def test_dgetrf_block_size_sweep(dev, nb, leaf):
+    """NB/leaf sweep through the XLAENV-style hook (TESTING/dtest.in:8-9 sweeps NB)."""
+    n = 200
+    a = la.matrix(2, 4242, n, n)
+    lu_r, p_r, _ = oracle.dgetrf(a)
+    la.set_tuning(nb, leaf)
+    try:
+        lu, p, info = gpu_getrf(a)
+    finally:
+        la.set_tuning(0, 0)
+    assert info == 0
+    assert_ipiv(p, p_r, lu_r)
+    assert oracle.dget01(a, lu, p) < THRESH
+
+
+@pytest.mark.parametrize("kind", ["first", "last", "half"])
+def test_dgetrf_zero_columns_info(dev, kind):
+    """DCHKGE types 5-7 (TESTING/LIN/dchkge.f:325-347): INFO is the first zero column."""
+    n = 90
+    a = la.matrix(2, 77, n, n)
+    if kind == "first":
+        a[:, 0] = 0.0
+        izero = 1
+    elif kind == "last":
+        a[:, n - 1] = 0.0
+        izero = n
+    else:
+        a[:, n // 2:] = 0.0
+        izero = n // 2 + 1
+    lu_r, p_r, info_r = oracle.dgetrf(a)
+    lu, p, info = gpu_getrf(a)
+    assert info_r == izero
+    assert info == izero
+    assert_ipiv(p, p_r, lu_r)
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (7, 3), (255, 1), (257, 15), (1024, 16)])
+def test_dgetrs_parity(dev, trans, n, nrhs):
+    a = la.matrix(2, 1000 + n, n, n)
+    b = la.matrix(2, 2000 + n, n, nrhs)
+    lu_r, p_r, _ = oracle.dgetrf(a)
+    lu, p, info = gpu_getrf(a)
+    assert info == 0
+    d_lu, d_p, d_b = to_dev(lu), to_dev(p.reshape(-1, 1).astype(np.int32)), to_dev(b)
+    assert la.dgetrs(trans, n, nrhs, d_lu, n, d_p, d_b, n) == 0
+    x = from_dev(d_b, (n, nrhs))
+    assert oracle.dget02(a, x, b, trans) < THRESH
+    x_r, _ = oracle.dgetrs(lu_r, p_r, b, trans)
+    # same LU up to rounding -> same solution up to cond(A)*eps
+    np.testing.assert_allclose(x, x_r, rtol=1e-9 * n, atol=1e-9 * n * np.abs(x_r).max())
+
+
+def test_dgesv_host_pointers(dev):
+    n, nrhs = 300, 4
+    a = la.matrix(2, 5, n, n)
+    b = la.matrix(2, 6, n, nrhs)
+    aa = np.array(a, order="F")
+    bb = np.array(b, order="F")
+    ipiv = np.zeros(n, dtype=np.int32)
+    info = la.dgesv(n, nrhs, aa, n, ipiv, bb, n)
+    assert info == 0
+    lu_r, p_r, _ = oracle.dgetrf(a)
+    assert_ipiv(ipiv, p_r, lu_r)
+    assert oracle.dget01(a, aa, ipiv) < THRESH
+    assert oracle.dget02(a, bb, b) < THRESH
+
+
+def test_dgetrf_host_with_lda(dev):
+    m, n, lda = 100, 80, 123
+    a = la.matrix(2, 8, m, n)
+    buf = np.full((lda, n), 7.0, order="F")
+    buf[:m, :] = a
+    ipiv = np.zeros(min(m, n), dtype=np.int32)
+    info = la.dgetrf(m, n, buf, lda, ipiv)
+    assert info == 0
+    assert np.all(buf[m:, :] == 7.0)  # rows beyond M untouched
+    lu_r, p_r, _ = oracle.dgetrf(a)
+    assert_ipiv(ipiv, p_r, lu_r)
+    assert oracle.dget01(a, np.asfortranarray(buf[:m, :]), ipiv) < THRESH
+
+
+@pytest.mark.parametrize("incx", [1, -1, 2, -2])
+def test_dlaswp_bitwise(dev, incx):
+    m, n = 60, 9
+    a = la.matrix(2, 9, m, n)
+    k1, k2 = 3, 40
+    rng = np.random.default_rng(incx + 10)
+    ipiv = np.zeros(k1 + (k2 - k1) * abs(incx) + 1, dtype=np.int32)
+    ipiv[:] = rng.integers(1, m + 1, size=len(ipiv))
+    ref = oracle.dlaswp(a, k1, k2, ipiv, incx)
+    if oracle.ref_available():
+        assert np.array_equal(ref, oracle.ref_dlaswp(a, k1, k2, ipiv, incx))
+    aa = np.array(a, order="F")
+    la.dlaswp(n, aa, m, k1, k2, ipiv, incx)
+    assert np.array_equal(aa, ref)
+
+
+@pytest.mark.parametrize("n", [64, 300, 1024])
+def test_sgetrf_parity(dev, n):
+    a = la.matrix(2, 3000 + n, n, n).astype(np.float32)
+    lu_r, p_r, _ = oracle.sgetrf(a)
+    lu, p, info = gpu_getrf(a, np.float32)
+    assert info == 0
+    k = first_mismatch(p, p_r)
+    if k >= 0:
+        assert near_tie(lu_r.astype(np.float64), k) < 1e-5
+    # SGET01-style ratio in single precision
+    l = np.tril(lu.astype(np.float64), -1) + np.eye(n)
+    u = np.triu(lu.astype(np.float64))
+    pa = oracle.dlaswp(a.astype(np.float64), 1, n, p, 1)
+    r = np.abs(l @ u - pa).sum(0).max() / (n * np.abs(a.astype(np.float64)).sum(0).max() * 2.0 ** -24)
+    assert r < THRESH
+
+
+def test_determinism(dev):
+    """Two runs give the same bits (LDS / cross-workgroup races would not)."""
+    n = 700
+    a = la.matrix(3, 31, n, n)
+    lu1, p1, _ = gpu_getrf(a)
+    lu2, p2, _ = gpu_getrf(a)
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(lu1.view(np.int64), lu2.view(np.int64))
