@@ -59,6 +59,7 @@ def _declare(L):
         "lapack_amd_sgetrs": (i, [ch, i, i, vp, i, vp, vp, i]),
         "lapack_amd_dgesv": (i, [i, i, vp, i, vp, vp, i]),
         "lapack_amd_dlaswp": (i, [i, vp, i, i, i, vp, i]),
+        "lapack_amd_dsgesv": (i, [i, i, vp, i, vp, vp, i, vp, i, _c_int_p]),
         "lapack_amd_dgemm": (i, [ch, ch, i, i, i, d, vp, i, vp, i, d, vp, i]),
         "lapack_amd_dgetrf_async": (i, [i, i, vp, i, vp, vp, vp]),
         "lapack_amd_sgetrf_async": (i, [i, i, vp, i, vp, vp, vp]),
@@ -136,6 +137,13 @@ def sgetrs(trans, n, nrhs, a, lda, ipiv, b, ldb):
 def dgesv(n, nrhs, a, lda, ipiv, b, ldb):
     """DGESV(N, NRHS, A, LDA, IPIV, B, LDB, INFO) (SRC/dgesv.f:148)."""
     return lib().lapack_amd_dgesv(n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb)
+
+
+def dsgesv(n, nrhs, a, lda, ipiv, b, ldb, x, ldx):
+    """DSGESV (SRC/dsgesv.f:191); returns (INFO, ITER)."""
+    it = ctypes.c_int(0)
+    info = lib().lapack_amd_dsgesv(n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb, _ptr(x), ldx, ctypes.byref(it))
+    return info, it.value
 
 
 def dlaswp(n, a, lda, k1, k2, ipiv, incx):

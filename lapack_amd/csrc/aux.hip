@@ -49,28 +49,92 @@ void launch_transpose_t(int rows, int cols, const Ti* in, long ld_in, To* out, l
 
 // ---- row interchanges in the transposed workspace ---------------------------
 // T = A^T: interchanging rows r and p of A is interchanging columns r and p of
-// T.  One thread per T row c in [r0, r1); the interchanges are applied in the
-// order of SRC/dlaswp.f:156-166 (k = k1..k2-1), so a row hit twice ends where
-// the reference puts it.  Consecutive threads touch consecutive addresses.
-template <class T>
-__global__ __launch_bounds__(256) void laswp_cols_kernel(T* __restrict__ Tm, long ldt, int r0, int r1,
-                                                         int k1, int k2, const int* __restrict__ ipiv) {
-  __shared__ int sp[512];
-  const int nk = k2 - k1;
-  for (int i = threadIdx.x; i < nk; i += 256) sp[i] = ipiv[k1 + i] - 1;
+// T.  The panel's interchange sequence ipiv[k1..k2) (SRC/dlaswp.f:156-166 order)
+// is first composed into (dst, src) pairs over the positions it touches --
+// one workgroup walks the sequence once over a position map in LDS -- and the
+// columns are then moved in a single pass: each workgroup owns 16 T rows (one
+// 128-byte line per T column), stages every source value in LDS, barriers,
+// and writes every destination.  Two row ranges (left and right of the panel)
+// are served by one launch.
+constexpr int SWP_ROWS = 16;      // T rows per workgroup
+constexpr int SWP_MAXPAIRS = 512; // 2 * the largest panel width
+
+// pairs[2q] = dst, pairs[2q+1] = src (0-based positions); *npairs = count.
+// Positions live in [k1, npos); map is LDS (or global when it does not fit).
+template <bool LDSMAP>
+__global__ __launch_bounds__(256) void swaps_to_pairs_kernel(int k1, int k2, int npos, const int* __restrict__ ipiv,
+                                                             int* __restrict__ pairs, int* __restrict__ npairs,
+                                                             int* __restrict__ gmap) {
+  extern __shared__ int smem[];
+  __shared__ int s_cnt;
+  int* cur = LDSMAP ? smem : gmap;
+  const int span = npos - k1;
+  for (int x = threadIdx.x; x < span; x += 256) cur[x] = k1 + x;
+  if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
-  const long c = (long)r0 + (long)blockIdx.x * 256 + threadIdx.x;
-  if (c >= r1) return;
-  for (int i = 0; i < nk; ++i) {
-    const int r = k1 + i, p = sp[i];
-    if (p != r) {
-      T* a = Tm + c + (long)r * ldt;
-      T* b = Tm + c + (long)p * ldt;
-      const T t = *a;
-      *a = *b;
-      *b = t;
+  if (threadIdx.x == 0) {
+    for (int i = k1; i < k2; ++i) {
+      const int p = ipiv[i] - 1;
+      if (p != i && p >= k1 && p < npos) {
+        const int t = cur[i - k1];
+        cur[i - k1] = cur[p - k1];
+        cur[p - k1] = t;
+      }
     }
   }
+  __syncthreads();
+  for (int x = threadIdx.x; x < span; x += 256) {
+    const int src = cur[x];
+    if (src != k1 + x) {
+      const int q = atomicAdd(&s_cnt, 1);
+      if (q < SWP_MAXPAIRS) {
+        pairs[2 * q] = k1 + x;
+        pairs[2 * q + 1] = src;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *npairs = min(s_cnt, SWP_MAXPAIRS);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void apply_pairs_cols_kernel(T* __restrict__ Tm, long ldt, int r0, int r1, int r2,
+                                                               int r3, const int* __restrict__ pairs,
+                                                               const int* __restrict__ npairs) {
+  __shared__ T st[SWP_MAXPAIRS * SWP_ROWS];
+  const int np = *npairs;
+  // this block's 16 rows: first from [r0, r1), then from [r2, r3)
+  const int n1 = (r1 - r0 + SWP_ROWS - 1) / SWP_ROWS;
+  const int blk = blockIdx.x;
+  const long c0 = blk < n1 ? (long)r0 + (long)blk * SWP_ROWS : (long)r2 + (long)(blk - n1) * SWP_ROWS;
+  const long cend = blk < n1 ? r1 : r3;
+  const int cl = threadIdx.x & (SWP_ROWS - 1);
+  const int qg = threadIdx.x / SWP_ROWS;  // 16 pair lanes
+  const long c = c0 + cl;
+  const bool ok = c < cend;
+  for (int q = qg; q < np; q += 256 / SWP_ROWS)
+    if (ok) st[q * SWP_ROWS + cl] = Tm[c + (long)pairs[2 * q + 1] * ldt];
+  __syncthreads();
+  for (int q = qg; q < np; q += 256 / SWP_ROWS)
+    if (ok) Tm[c + (long)pairs[2 * q] * ldt] = st[q * SWP_ROWS + cl];
+}
+
+// Row interchanges of a column-major matrix through the same pairs:
+// B(dst, j) <- B_old(src, j); one workgroup per column block of 16.
+template <class T>
+__global__ __launch_bounds__(256) void apply_pairs_rows_kernel(T* __restrict__ B, long ldb, int ncols,
+                                                               const int* __restrict__ pairs,
+                                                               const int* __restrict__ npairs) {
+  __shared__ T st[SWP_MAXPAIRS * SWP_ROWS];
+  const int np = *npairs;
+  const int j = blockIdx.x * SWP_ROWS + (threadIdx.x & (SWP_ROWS - 1));
+  const int qg = threadIdx.x / SWP_ROWS;
+  const bool ok = j < ncols;
+  for (int q = qg; q < np; q += 256 / SWP_ROWS)
+    if (ok) st[q * SWP_ROWS + (threadIdx.x & (SWP_ROWS - 1))] = B[(long)pairs[2 * q + 1] + (long)j * ldb];
+  __syncthreads();
+  for (int q = qg; q < np; q += 256 / SWP_ROWS)
+    if (ok) B[(long)pairs[2 * q] + (long)j * ldb] = st[q * SWP_ROWS + (threadIdx.x & (SWP_ROWS - 1))];
 }
 
 // ---- DLASWP on a column-major matrix (row interchanges) ----------------------
@@ -217,23 +281,7 @@ void launch_transpose_d2s(int rows, int cols, const double* in, long ld_in, floa
   launch_transpose_t<double, float, true>(rows, cols, in, ld_in, out, ld_out, overflow, s);
 }
 
-void launch_laswp_cols_d(double* T, long ldt, int r0, int r1, int k1, int k2, const int* ipiv,
-                         hipStream_t s) {
-  for (int kb = k1; kb < k2; kb += 512) {
-    const int ke = kb + 512 < k2 ? kb + 512 : k2;
-    if (r1 > r0) laswp_cols_kernel<double><<<(r1 - r0 + 255) / 256, 256, 0, s>>>(T, ldt, r0, r1, kb, ke, ipiv);
-  }
-}
-void launch_laswp_cols_s(float* T, long ldt, int r0, int r1, int k1, int k2, const int* ipiv,
-                         hipStream_t s) {
-  for (int kb = k1; kb < k2; kb += 512) {
-    const int ke = kb + 512 < k2 ? kb + 512 : k2;
-    if (r1 > r0) laswp_cols_kernel<float><<<(r1 - r0 + 255) / 256, 256, 0, s>>>(T, ldt, r0, r1, kb, ke, ipiv);
-  }
-}
-
-// perm (nrows ints) <- permutation of the interchange sequence; see above.
-void launch_swaps_to_perm(int nrows, int k1, int k2, const int* ipiv, int incx, int* perm, hipStream_t s) {
+static int lds_limit() {
   static int lds_max = -1;
   if (lds_max < 0) {
     int dev = 0, v = 65536;
@@ -242,8 +290,48 @@ void launch_swaps_to_perm(int nrows, int k1, int k2, const int* ipiv, int incx, 
       v = 65536;
     lds_max = v;
   }
+  return lds_max;
+}
+
+int swp_max_pairs() { return SWP_MAXPAIRS; }
+
+// (dst, src) pairs of the interchange sequence ipiv[k1..k2) over positions
+// [k1, npos); gmap: npos ints of scratch used when the map does not fit LDS.
+void launch_swaps_to_pairs(int k1, int k2, int npos, const int* ipiv, int* pairs, int* npairs, int* gmap,
+                           hipStream_t s) {
+  const size_t need = sizeof(int) * (size_t)(npos - k1) + 64;
+  if (need <= (size_t)lds_limit())
+    swaps_to_pairs_kernel<true><<<1, 256, sizeof(int) * (size_t)(npos - k1), s>>>(k1, k2, npos, ipiv, pairs, npairs, gmap);
+  else
+    swaps_to_pairs_kernel<false><<<1, 256, 0, s>>>(k1, k2, npos, ipiv, pairs, npairs, gmap);
+}
+template <class T>
+static void apply_cols_t(T* Tm, long ldt, int r0, int r1, int r2, int r3, const int* pairs, const int* npairs,
+                         hipStream_t s) {
+  if (r1 < r0) r1 = r0;
+  if (r3 < r2) r3 = r2;
+  const int nb = (r1 - r0 + SWP_ROWS - 1) / SWP_ROWS + (r3 - r2 + SWP_ROWS - 1) / SWP_ROWS;
+  if (nb > 0) apply_pairs_cols_kernel<T><<<nb, 256, 0, s>>>(Tm, ldt, r0, r1, r2, r3, pairs, npairs);
+}
+void launch_apply_pairs_cols_d(double* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs,
+                               const int* npairs, hipStream_t s) {
+  apply_cols_t<double>(T, ldt, r0, r1, r2, r3, pairs, npairs, s);
+}
+void launch_apply_pairs_cols_s(float* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs,
+                               const int* npairs, hipStream_t s) {
+  apply_cols_t<float>(T, ldt, r0, r1, r2, r3, pairs, npairs, s);
+}
+void launch_apply_pairs_rows_d(double* B, long ldb, int ncols, const int* pairs, const int* npairs, hipStream_t s) {
+  if (ncols > 0) apply_pairs_rows_kernel<double><<<(ncols + SWP_ROWS - 1) / SWP_ROWS, 256, 0, s>>>(B, ldb, ncols, pairs, npairs);
+}
+void launch_apply_pairs_rows_s(float* B, long ldb, int ncols, const int* pairs, const int* npairs, hipStream_t s) {
+  if (ncols > 0) apply_pairs_rows_kernel<float><<<(ncols + SWP_ROWS - 1) / SWP_ROWS, 256, 0, s>>>(B, ldb, ncols, pairs, npairs);
+}
+
+// perm (nrows ints) <- permutation of the interchange sequence; see above.
+void launch_swaps_to_perm(int nrows, int k1, int k2, const int* ipiv, int incx, int* perm, hipStream_t s) {
   const size_t need = sizeof(int) * (PERM_CHUNK + (size_t)nrows);
-  if (nrows <= PERM_LDS_MAX && need <= (size_t)lds_max) {
+  if (nrows <= PERM_LDS_MAX && need <= (size_t)lds_limit()) {
     const size_t sh = sizeof(int) * (PERM_CHUNK + (size_t)nrows);
     swaps_to_perm_kernel<true><<<1, 256, sh, s>>>(nrows, k1, k2, ipiv, incx, perm);
   } else {

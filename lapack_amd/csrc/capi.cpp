@@ -3,7 +3,8 @@
 //
 // Argument checking follows the reference routine by routine (order and
 // parameter numbers): DGETRF SRC/dgetrf.f:144-155, DGETRF2 SRC/dgetrf2.f:152-163,
-// DGETRS SRC/dgetrs.f:157-174, DGESV SRC/dgesv.f:148-162, DGEMM
+// DGETRS SRC/dgetrs.f:157-174, DGESV SRC/dgesv.f:148-162, DSGESV
+// SRC/dsgesv.f:263-280, DGEMM
 // BLAS/SRC/dgemm.f:271-294, DLASWP none (SRC/dlaswp.f:138-150).  The
 // Fortran symbols call XERBLA with the routine name and -INFO and return,
 // like the reference; the lapack_amd_* entry points return -INFO instead.
@@ -36,6 +37,8 @@ bool is_device_ptr(const void* p);
 void phase_enable(int on);
 void phase_read(double* ms, long* calls, double* flops);
 void phase_reset();
+int dsgesv_device(int n, int nrhs, double* A, long lda, int* ipiv, const double* B, long ldb, double* X, long ldx,
+                  int* iter, hipStream_t s);
 }  // namespace la
 
 using namespace la;
@@ -230,6 +233,38 @@ int gesv_sync(int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb
   return info;
 }
 
+// DSGESV: arguments checked as SRC/dsgesv.f:263-280.
+int dsgesv_sync(int n, int nrhs, double* a, int lda, int* ipiv, const double* b, int ldb, double* x, int ldx,
+                int* iter) {
+  *iter = 0;
+  if (n < 0) return -1;
+  if (nrhs < 0) return -2;
+  if (lda < std::max(1, n)) return -4;
+  if (ldb < std::max(1, n)) return -7;
+  if (ldx < std::max(1, n)) return -9;
+  if (n == 0) return 0;
+  Call call;
+  int rc = call.open();
+  if (rc) return rc;
+  Arr<double> A = arr<double>(a, lda, n, n);
+  Arr<int> P = arr<int>(ipiv, n, n, 1);
+  Arr<double> B = arr<double>(const_cast<double*>(b), ldb, n, nrhs);
+  Arr<double> X = arr<double>(x, ldx, n, nrhs);
+  char* io = nullptr;
+  if ((rc = io_buffer(call.c, A.bytes() + P.bytes() + B.bytes() + X.bytes() + 256, &io))) return rc;
+  char* cur = io;
+  place(A, cur);
+  place(P, cur);
+  place(B, cur);
+  place(X, cur);
+  if ((rc = to_dev(A, call.s)) || (rc = to_dev(B, call.s))) return rc;
+  const int info = dsgesv_device(n, nrhs, A.dev, A.ld, P.dev, B.dev, B.ld, X.dev, X.ld, iter, call.s);
+  if (info < 0) return info;
+  if ((rc = to_host(A, call.s)) || (rc = to_host(P, call.s)) || (rc = to_host(X, call.s))) return rc;
+  CK(hipStreamSynchronize(call.s));
+  return info;
+}
+
 template <class T>
 int laswp_sync(int n, T* a, int lda, int k1, int k2, const int* ipiv, int incx) {
   if (n <= 0 || incx == 0) return 0;
@@ -384,6 +419,11 @@ void dgesv_(const int* n, const int* nrhs, double* a, const int* lda, int* ipiv,
   *info = gesv_sync(*n, *nrhs, a, *lda, ipiv, b, *ldb);
   if (*info < 0 && *info > -1000) call_xerbla("DGESV ", *info);
 }
+void dsgesv_(const int* n, const int* nrhs, double* a, const int* lda, int* ipiv, const double* b,
+             const int* ldb, double* x, const int* ldx, double*, float*, int* iter, int* info) {
+  *info = dsgesv_sync(*n, *nrhs, a, *lda, ipiv, b, *ldb, x, *ldx, iter);
+  if (*info < 0 && *info > -1000) call_xerbla("DSGESV", *info);
+}
 void dlaswp_(const int* n, double* a, const int* lda, const int* k1, const int* k2, const int* ipiv,
              const int* incx) {
   (void)laswp_sync<double>(*n, a, *lda, *k1, *k2, ipiv, *incx);
@@ -406,6 +446,10 @@ int lapack_amd_sgetrs(char trans, int n, int nrhs, const float* a, int lda, cons
 }
 int lapack_amd_dgesv(int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb) {
   return gesv_sync(n, nrhs, a, lda, ipiv, b, ldb);
+}
+int lapack_amd_dsgesv(int n, int nrhs, double* a, int lda, int* ipiv, const double* b, int ldb, double* x,
+                      int ldx, int* iter) {
+  return dsgesv_sync(n, nrhs, a, lda, ipiv, b, ldb, x, ldx, iter);
 }
 int lapack_amd_dlaswp(int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx) {
   return laswp_sync<double>(n, a, lda, k1, k2, ipiv, incx);

@@ -49,21 +49,27 @@ template <> struct Mfma<float> {
   static constexpr int APAD = 16;
 };
 
-template <class T, bool TA, bool TB>
+// UPD: the trailing-update form C <- C - A*B (alpha = -1, beta = 1), a
+// separate symbol so profiles can single out the DGETRF trailing update.
+template <class T, bool TA, bool TB, bool UPD>
 __global__ __launch_bounds__(NT, 2) void gemm_kernel(int M, int N, int K, T alpha,
                                                      const T* __restrict__ A, long lda,
                                                      const T* __restrict__ B, long ldb, T beta,
                                                      T* __restrict__ C, long ldc, int tm, int tn,
-                                                     int chunk) {
+                                                     int chunk, int nxcd) {
   typedef typename Mfma<T>::acc_t acc_t;
   constexpr int AST = BM + Mfma<T>::APAD;  // As row stride
   constexpr int BST = BK + 1;              // Bs row stride
   __shared__ T As[2][BK * AST];
   __shared__ T Bs[2][BN * BST];
 
-  // XCD-aware tile selection.
+  // XCD-aware tile selection; with nxcd == 7 XCD 0 is left idle.
   const int bid = blockIdx.x;
-  const int xcd = bid % NXCD;
+  int xcd = bid % NXCD;
+  if (nxcd < NXCD) {
+    if (xcd < NXCD - nxcd) return;
+    xcd -= NXCD - nxcd;
+  }
   const int lin = xcd * chunk + bid / NXCD;
   if (lin >= tm * tn) return;
   const int width = GROUP_M * tn;
@@ -174,7 +180,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(int M, int N, int K, T alph
         if (gr < M) {
           T* cp = C + (long)gr + (long)gc * ldc;
           const T v = acc[i][j][r];
-          *cp = (beta == T(0)) ? alpha * v : alpha * v + beta * *cp;
+          if (UPD) *cp = *cp - v;
+          else *cp = (beta == T(0)) ? alpha * v : alpha * v + beta * *cp;
         }
       }
     }
@@ -194,13 +201,24 @@ void launch_gemm_t(char ta, char tb, int m, int n, int k, T alpha, const T* A, l
   const bool TB = (tb == 'T' || tb == 't' || tb == 'C' || tb == 'c');
   if (k <= 0 || alpha == T(0)) k = 0;  // C <- beta*C (dgemm.f:303-318)
   if (!TA && !TB)
-    gemm_kernel<T, false, false><<<grid, NT, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, chunk);
+    gemm_kernel<T, false, false, false><<<grid, NT, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, chunk, NXCD);
   else if (!TA && TB)
-    gemm_kernel<T, false, true><<<grid, NT, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, chunk);
+    gemm_kernel<T, false, true, false><<<grid, NT, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, chunk, NXCD);
   else if (TA && !TB)
-    gemm_kernel<T, true, false><<<grid, NT, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, chunk);
+    gemm_kernel<T, true, false, false><<<grid, NT, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, chunk, NXCD);
   else
-    gemm_kernel<T, true, true><<<grid, NT, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, chunk);
+    gemm_kernel<T, true, true, false><<<grid, NT, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, chunk, NXCD);
+}
+
+template <class T>
+void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long ldb, T* C, long ldc, bool skip0,
+                     hipStream_t s) {
+  if (m <= 0 || n <= 0 || k <= 0) return;
+  const int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+  const int nx = skip0 ? NXCD - 1 : NXCD;
+  const int chunk = (tm * tn + nx - 1) / nx;
+  gemm_kernel<T, false, false, true><<<dim3(chunk * NXCD), NT, 0, s>>>(m, n, k, T(-1), A, lda, B, ldb, T(1), C, ldc,
+                                                                     tm, tn, chunk, nx);
 }
 
 }  // namespace
@@ -208,6 +226,14 @@ void launch_gemm_t(char ta, char tb, int m, int n, int k, T alpha, const T* A, l
 void launch_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double* A, long lda,
                   const double* B, long ldb, double beta, double* C, long ldc, hipStream_t s) {
   launch_gemm_t<double>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+}
+void launch_dgemm_update(int m, int n, int k, const double* A, long lda, const double* B, long ldb, double* C,
+                         long ldc, bool skip0, hipStream_t s) {
+  launch_update_t<double>(m, n, k, A, lda, B, ldb, C, ldc, skip0, s);
+}
+void launch_sgemm_update(int m, int n, int k, const float* A, long lda, const float* B, long ldb, float* C, long ldc,
+                         bool skip0, hipStream_t s) {
+  launch_update_t<float>(m, n, k, A, lda, B, ldb, C, ldc, skip0, s);
 }
 void launch_sgemm(char ta, char tb, int m, int n, int k, float alpha, const float* A, long lda,
                   const float* B, long ldb, float beta, float* C, long ldc, hipStream_t s) {

@@ -37,6 +37,12 @@ __device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); 
 // C <- alpha*op(A)*op(B) + beta*C, column-major, FP64 MFMA.
 void launch_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double* A, long lda,
                   const double* B, long ldb, double beta, double* C, long ldc, hipStream_t s);
+// The DGETRF trailing update C <- C - A*B (its own kernel symbol).
+// skip_xcd0: leave XCD 0 to a concurrent panel kernel.
+void launch_dgemm_update(int m, int n, int k, const double* A, long lda, const double* B, long ldb, double* C,
+                         long ldc, bool skip_xcd0, hipStream_t s);
+void launch_sgemm_update(int m, int n, int k, const float* A, long lda, const float* B, long ldb, float* C, long ldc,
+                         bool skip_xcd0, hipStream_t s);
 // Same in FP32 (MFMA f32).
 void launch_sgemm(char ta, char tb, int m, int n, int k, float alpha, const float* A, long lda,
                   const float* B, long ldb, float beta, float* C, long ldc, hipStream_t s);
@@ -50,13 +56,19 @@ void launch_transpose_s(int rows, int cols, const float* in, long ld_in, float* 
 void launch_transpose_d2s(int rows, int cols, const double* in, long ld_in, float* out, long ld_out,
                           int* overflow, hipStream_t s);
 
-// Interchange columns r <-> ipiv[r]-1 (sequentially for r = k1..k2-1, 0-based
-// k's, 1-based ipiv) of T, restricted to rows [r0, r1).  In the transposed
-// workspace T = A^T this is DLASWP on A's columns r0..r1-1.
-void launch_laswp_cols_d(double* T, long ldt, int r0, int r1, int k1, int k2, const int* ipiv,
-                         hipStream_t s);
-void launch_laswp_cols_s(float* T, long ldt, int r0, int r1, int k1, int k2, const int* ipiv,
-                         hipStream_t s);
+// Row interchanges in the transposed workspace T = A^T (A's rows are T's
+// columns): the sequence ipiv[k1..k2) (1-based entries, applied in order)
+// is composed into at most swp_max_pairs() (dst, src) position pairs over
+// [k1, npos), then applied to T rows [r0, r1) and [r2, r3) in one pass.
+int swp_max_pairs();
+void launch_swaps_to_pairs(int k1, int k2, int npos, const int* ipiv, int* pairs, int* npairs, int* gmap,
+                           hipStream_t s);
+void launch_apply_pairs_cols_d(double* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs,
+                               const int* npairs, hipStream_t s);
+void launch_apply_pairs_cols_s(float* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs,
+                               const int* npairs, hipStream_t s);
+void launch_apply_pairs_rows_d(double* B, long ldb, int ncols, const int* pairs, const int* npairs, hipStream_t s);
+void launch_apply_pairs_rows_s(float* B, long ldb, int ncols, const int* pairs, const int* npairs, hipStream_t s);
 // DLASWP on a column-major matrix: perm (nrows ints) <- the permutation of
 // the interchange sequence ipiv[k1..k2-1] (1-based entries, incx = +1/-1),
 // then rows are gathered through it.
@@ -87,17 +99,28 @@ void launch_fill_i32(int* p, size_t n, int v, hipStream_t s);
 // info <- LA_ERR_SYNC when a panel barrier timed out (err != 0).
 void launch_fold_err(const int* err, int* info, hipStream_t s);
 
-// Cooperative panel (leaf) factorization on the transposed workspace: see panel.hip.
-struct PanelSync {
-  unsigned* counters;  // one barrier counter per launch (pre-zeroed)
-  double* xchg;        // exchange area (candidates and rows), device memory
-  int* err;            // set to 1 when a barrier wait timed out
-};
+// Cooperative panel (leaf) factorization on the transposed workspace: see
+// panel.hip.  tags (panel_tag_count() unsigned) must be zero at the start of
+// a factorization and tagbase must grow by >= 64 from one launch to the next.
 int panel_max_width(int h);  // widest leaf the panel kernel takes at height h
 size_t panel_xchg_bytes(int max_groups);
-void launch_getf2_panel_d(double* T, long ldt, int j0, int w, int m, int* ipiv, int* info,
-                          unsigned* counter, double* xchg, int* err, hipStream_t s);
-void launch_getf2_panel_s(float* T, long ldt, int j0, int w, int m, int* ipiv, int* info,
-                          unsigned* counter, double* xchg, int* err, hipStream_t s);
+size_t panel_tag_count();
+int panel_groups(int h);     // workgroups used at height h (0 = one workgroup, LDS only)
+// rankctr: a zeroed counter private to this launch; xcd0: run the
+// multi-workgroup form on XCD 0 only (when it fits one workgroup per CU).
+int launch_getf2_panel_d(double* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned tagbase,
+                         double* xchg, unsigned* tags, int* err, unsigned* rankctr, bool xcd0, hipStream_t s);
+int launch_getf2_panel_s(float* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned tagbase,
+                         double* xchg, unsigned* tags, int* err, unsigned* rankctr, bool xcd0, hipStream_t s);
+
+// Mixed-precision helpers (mp.hip).
+void launch_cast_d2s(int m, int n, const double* A, long lda, float* S, long lds, int* ovf, hipStream_t s);
+void launch_cast_s2d(int m, int n, const float* S, long lds, double* A, long lda, hipStream_t s);
+void launch_axpy(int m, int n, const double* X, long ldx, double* Y, long ldy, hipStream_t s);
+void launch_col_absmax(int m, int n, const double* A, long lda, unsigned long long* out, hipStream_t s);
+void launch_row_abssum(int m, int n, const double* A, long lda, double* rowsum, hipStream_t s);
+void launch_row_absmax(int m, int n, const double* A, long lda, double* r, hipStream_t s);
+void launch_col_absmax_scaled(int m, int n, const double* A, long lda, const double* r, double* c, hipStream_t s);
+void launch_scale_rc(int m, int n, double* A, long lda, const double* r, const double* c, int mode, hipStream_t s);
 
 }  // namespace la

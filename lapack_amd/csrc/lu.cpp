@@ -46,8 +46,14 @@ struct DevCtx {
   size_t io_bytes = 0;
   hipEvent_t ws_free = nullptr;  // recorded after each use of ws
   bool ws_used = false;
+  hipStream_t pstream = nullptr;      // panel stream (look-ahead)
+  hipStream_t ustream = nullptr;      // trailing-update stream
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  char* mp = nullptr;            // buffers of the mixed-precision solvers
+  size_t mp_bytes = 0;
   std::mutex mu;     // guards ws and the launch sequence of one call
   std::mutex io_mu;  // guards io across a whole synchronous call
+  std::mutex mp_mu;  // guards mp across a whole mixed-precision solve
 };
 
 static std::mutex g_mu;
@@ -141,6 +147,17 @@ static int io_acquire(DevCtx* c, size_t bytes, char** out) {
   return 0;
 }
 
+static int mp_acquire(DevCtx* c, size_t bytes, char** out) {
+  if (bytes > c->mp_bytes) {
+    if (c->mp) { LA_TRY(hipDeviceSynchronize()); LA_TRY(hipFree(c->mp)); c->mp = nullptr; c->mp_bytes = 0; }
+    size_t want = align_up(bytes, (size_t)1 << 21);
+    if (hipMalloc(&c->mp, want) != hipSuccess) { (void)hipGetLastError(); t_last_error = LA_ERR_NOMEM; return LA_ERR_NOMEM; }
+    c->mp_bytes = want;
+  }
+  *out = c->mp;
+  return 0;
+}
+
 // Carves aligned sub-buffers out of one allocation.
 struct Carve {
   size_t off = 0;
@@ -167,11 +184,16 @@ template <> struct K<double> {
                    long ldb, double be, double* C, long ldc, hipStream_t s) {
     launch_dgemm(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
   }
+  static void update(int m, int n, int k, const double* A, long lda, const double* B, long ldb, double* C, long ldc,
+                     bool skip0, hipStream_t s) {
+    launch_dgemm_update(m, n, k, A, lda, B, ldb, C, ldc, skip0, s);
+  }
   static void transpose(int r, int c, const double* in, long li, double* out, long lo, hipStream_t s) {
     launch_transpose_d(r, c, in, li, out, lo, s);
   }
-  static void laswp_cols(double* T, long ldt, int r0, int r1, int k1, int k2, const int* ipiv, hipStream_t s) {
-    launch_laswp_cols_d(T, ldt, r0, r1, k1, k2, ipiv, s);
+  static void apply_cols(double* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs, const int* np,
+                         hipStream_t s) {
+    launch_apply_pairs_cols_d(T, ldt, r0, r1, r2, r3, pairs, np, s);
   }
   static void gather(int nr, int nc, const int* perm, const double* src, long ls, double* dst, long ld, hipStream_t s) {
     launch_gather_rows_d(nr, nc, perm, src, ls, dst, ld, s);
@@ -183,9 +205,9 @@ template <> struct K<double> {
   static void copy(int r, int c, const double* src, long ls, double* dst, long ld, hipStream_t s) {
     launch_copy_d(r, c, src, ls, dst, ld, s);
   }
-  static void panel(double* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned* ctr, double* x,
-                    int* err, hipStream_t s) {
-    launch_getf2_panel_d(T, ldt, j0, w, m, ipiv, info, ctr, x, err, s);
+  static int panel(double* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned tagbase, double* x,
+                   unsigned* tags, int* err, unsigned* rk, bool xcd0, hipStream_t s) {
+    return launch_getf2_panel_d(T, ldt, j0, w, m, ipiv, info, tagbase, x, tags, err, rk, xcd0, s);
   }
 };
 template <> struct K<float> {
@@ -193,11 +215,16 @@ template <> struct K<float> {
                    long ldb, float be, float* C, long ldc, hipStream_t s) {
     launch_sgemm(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
   }
+  static void update(int m, int n, int k, const float* A, long lda, const float* B, long ldb, float* C, long ldc,
+                     bool skip0, hipStream_t s) {
+    launch_sgemm_update(m, n, k, A, lda, B, ldb, C, ldc, skip0, s);
+  }
   static void transpose(int r, int c, const float* in, long li, float* out, long lo, hipStream_t s) {
     launch_transpose_s(r, c, in, li, out, lo, s);
   }
-  static void laswp_cols(float* T, long ldt, int r0, int r1, int k1, int k2, const int* ipiv, hipStream_t s) {
-    launch_laswp_cols_s(T, ldt, r0, r1, k1, k2, ipiv, s);
+  static void apply_cols(float* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs, const int* np,
+                         hipStream_t s) {
+    launch_apply_pairs_cols_s(T, ldt, r0, r1, r2, r3, pairs, np, s);
   }
   static void gather(int nr, int nc, const int* perm, const float* src, long ls, float* dst, long ld, hipStream_t s) {
     launch_gather_rows_s(nr, nc, perm, src, ls, dst, ld, s);
@@ -209,9 +236,9 @@ template <> struct K<float> {
   static void copy(int r, int c, const float* src, long ls, float* dst, long ld, hipStream_t s) {
     launch_copy_s(r, c, src, ls, dst, ld, s);
   }
-  static void panel(float* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned* ctr, double* x,
-                    int* err, hipStream_t s) {
-    launch_getf2_panel_s(T, ldt, j0, w, m, ipiv, info, ctr, x, err, s);
+  static int panel(float* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned tagbase, double* x,
+                   unsigned* tags, int* err, unsigned* rk, bool xcd0, hipStream_t s) {
+    return launch_getf2_panel_s(T, ldt, j0, w, m, ipiv, info, tagbase, x, tags, err, rk, xcd0, s);
   }
 };
 
@@ -237,49 +264,71 @@ static double g_phase_ms[LA_PHASE_COUNT];
 static long g_phase_calls[LA_PHASE_COUNT];
 static double g_phase_flops[LA_PHASE_COUNT];
 static std::mutex g_phase_mu;
+struct PendingPhases {
+  std::vector<hipEvent_t> ev;
+  std::vector<int> tag;
+  std::vector<double> flops;
+};
+static std::vector<PendingPhases> g_pending;
 
+// Hand the call's events over; they are resolved (synchronising on them)
+// only when the phase totals are read, so timing adds no host sync.
 static void phase_collect(PhaseTimer& pt, const std::vector<double>& flops) {
   if (!pt.on) return;
-  hipEventSynchronize(pt.ev.back());
   std::lock_guard<std::mutex> g(g_phase_mu);
-  for (size_t i = 0; i + 1 < pt.ev.size(); ++i) {
-    float ms = 0;
-    hipEventElapsedTime(&ms, pt.ev[i], pt.ev[i + 1]);
-    const int t = pt.tag[i];
-    if (t >= 0 && t < LA_PHASE_COUNT) {
-      g_phase_ms[t] += ms;
-      g_phase_calls[t] += 1;
-      g_phase_flops[t] += flops[i];
-    }
-  }
-  for (auto e : pt.ev) hipEventDestroy(e);
+  g_pending.push_back(PendingPhases{pt.ev, pt.tag, flops});
   pt.ev.clear();
   pt.tag.clear();
+}
+static void phase_resolve_locked() {
+  for (auto& pp : g_pending) {
+    if (pp.ev.empty()) continue;
+    hipEventSynchronize(pp.ev.back());
+    for (size_t i = 0; i + 1 < pp.ev.size(); ++i) {
+      float ms = 0;
+      hipEventElapsedTime(&ms, pp.ev[i], pp.ev[i + 1]);
+      const int t = pp.tag[i];
+      if (t >= 0 && t < LA_PHASE_COUNT) {
+        g_phase_ms[t] += ms;
+        g_phase_calls[t] += 1;
+        g_phase_flops[t] += pp.flops[i];
+      }
+    }
+    for (auto e : pp.ev) hipEventDestroy(e);
+  }
+  g_pending.clear();
 }
 
 template <class T>
 size_t getrf_ws_bytes(int m, int n, int nb, int leaf) {
+  (void)leaf;
   const long ldt = (long)align_up((size_t)n, 32);
   Carve cv;
   cv.take<T>((size_t)ldt * m);                    // T workspace
-  cv.take<T>((size_t)ldt * nb);                   // TRSM output
-  cv.take<T>((size_t)nb * nb);                    // triangle inverse
-  const int mn = std::min(m, n);
-  cv.take<unsigned>((size_t)(mn / std::max(1, leaf) + mn / std::max(1, nb) + 8));  // barrier counters
+  for (int i = 0; i < 2; ++i) {                   // per stream: TRSM output, triangle inverse, pairs
+    cv.take<T>((size_t)ldt * nb);
+    cv.take<T>((size_t)nb * nb);
+    cv.take<int>((size_t)2 * swp_max_pairs() + 64);
+  }
+  cv.take<unsigned>(panel_tag_count());           // panel exchange tags
   cv.take<char>(panel_xchg_bytes(0));
   cv.take<int>(4);                                // err flag
+  cv.take<int>((size_t)m);                        // position map (fallback)
+  cv.take<unsigned>((size_t)std::min(m, n) + 8);  // per-leaf rank counters
   return cv.off;
 }
 
 // Factor A (m x n, lda, device) in place; d_ipiv (min(m,n)) and d_info on
-// device.  Asynchronous on s.
+// device.  Asynchronous on s.  With look-ahead: the trailing update of block
+// j first updates the columns of panel j+1, which is then factored on the
+// device's panel stream while the rest of the update runs on s.
 template <class T>
 int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t s) {
   DevCtx* c = get_ctx();
   if (!c) return LA_ERR_NODEVICE;
   std::lock_guard<std::mutex> g(c->mu);
   if (m == 0 || n == 0) return 0;
-  if (m > 65536) return LA_ERR_LIMIT;  // panel kernel: <= 128 workgroups x 512 rows
+  if (m > 32768) return LA_ERR_LIMIT;  // panel kernel: <= 64 workgroups x 512 rows
   const int nb = tune_nb();
   const int mn = std::min(m, n);
   const long ldt = (long)align_up((size_t)n, 32);
@@ -287,86 +336,194 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
   char* ws = nullptr;
   int rc = ws_acquire(c, getrf_ws_bytes<T>(m, n, nb, tune_leaf()), s, &ws);
   if (rc) return rc;
+  if (!c->pstream) {
+    // The panel chain is latency-bound and runs beside the trailing update:
+    // give it CUs of its own (LAPACK_AMD_PANEL_CUS per device, 0 = share all).
+    int ncu = 0;
+    LA_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
+    const int pcu = std::min(std::max(env_int("LAPACK_AMD_PANEL_CUS", 0), 0), ncu / 2);
+    if (pcu > 0 && ncu <= 1024) {
+      std::vector<uint32_t> mp((ncu + 31) / 32, 0u), mu((ncu + 31) / 32, 0u);
+      // spread the panel's CUs evenly over the CU index space (all XCDs)
+      const int stride = ncu / pcu;
+      for (int i = 0; i < ncu; ++i) {
+        const bool panel = (i % stride == 0) && (i / stride < pcu);
+        (panel ? mp : mu)[i / 32] |= 1u << (i % 32);
+      }
+      if (hipExtStreamCreateWithCUMask(&c->pstream, (uint32_t)mp.size(), mp.data()) != hipSuccess ||
+          hipExtStreamCreateWithCUMask(&c->ustream, (uint32_t)mu.size(), mu.data()) != hipSuccess) {
+        // CU-masked queues refused by the runtime: fall back to plain streams
+        (void)hipGetLastError();
+        if (c->pstream) hipStreamDestroy(c->pstream);
+        c->pstream = nullptr;
+        c->ustream = nullptr;
+      }
+    }
+    if (!c->pstream) {
+      LA_TRY(hipStreamCreateWithFlags(&c->pstream, hipStreamNonBlocking));
+      LA_TRY(hipStreamCreateWithFlags(&c->ustream, hipStreamNonBlocking));
+    }
+    for (int i = 0; i < 4; ++i) LA_TRY(hipEventCreateWithFlags(&c->ev[i], hipEventDisableTiming));
+  }
+  hipStream_t sp = c->pstream;
+  hipStream_t s_user = s;
+  // all work of this call runs on the library's two streams, fenced to the
+  // caller's stream at entry and exit
+  LA_TRY(hipEventRecord(c->ev[2], s_user));
+  s = c->ustream;
+  LA_TRY(hipStreamWaitEvent(s, c->ev[2], 0));
   Carve cv;
   T* Tm = reinterpret_cast<T*>(ws + cv.take<T>((size_t)ldt * m));
-  T* X = reinterpret_cast<T*>(ws + cv.take<T>((size_t)ldx * nb));
-  T* W = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nb * nb));
-  const size_t nctr = (size_t)(mn / std::max(1, tune_leaf()) + mn / nb + 8);
-  unsigned* ctr = reinterpret_cast<unsigned*>(ws + cv.take<unsigned>(nctr));
+  T* Xs[2];
+  T* Ws[2];
+  int* Ps[2];
+  for (int i = 0; i < 2; ++i) {
+    Xs[i] = reinterpret_cast<T*>(ws + cv.take<T>((size_t)ldx * nb));
+    Ws[i] = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nb * nb));
+    Ps[i] = reinterpret_cast<int*>(ws + cv.take<int>((size_t)2 * swp_max_pairs() + 64));
+  }
+  unsigned* tags = reinterpret_cast<unsigned*>(ws + cv.take<unsigned>(panel_tag_count()));
   double* xchg = reinterpret_cast<double*>(ws + cv.take<char>(panel_xchg_bytes(0)));
   int* err = reinterpret_cast<int*>(ws + cv.take<int>(4));
+  int* gmap = reinterpret_cast<int*>(ws + cv.take<int>((size_t)m));
+  const size_t nrank = (size_t)mn + 8;
+  unsigned* rankctr = reinterpret_cast<unsigned*>(ws + cv.take<unsigned>(nrank));
   const long ldw = nb;
+  const bool xcdsplit = env_int("LAPACK_AMD_PANEL_XCD", 1) != 0;
 
-  PhaseTimer pt;
-  pt.on = g_phase_on.load() != 0;
-  pt.s = s;
-  std::vector<double> fl;
-  auto mark = [&](int tag, double flops) {
+  PhaseTimer pa, pp;  // main stream, panel stream
+  pa.on = pp.on = g_phase_on.load() != 0;
+  pa.s = s;
+  pp.s = sp;
+  std::vector<double> fla, flp;
+  // Phase intervals: begin(tag) ... end() around the launches of one phase,
+  // so time spent waiting on the other stream is not attributed to it.
+  auto mark = [&](PhaseTimer& pt, std::vector<double>& fl, int tag, double flops) {
     if (!pt.on) return;
     pt.mark(tag);
     fl.push_back(flops);
   };
-
-  launch_fill_u32(ctr, nctr, 0u, s);
-  launch_fill_i32(err, 1, 0, s);
-  launch_fill_i32(info, 1, 0, s);
-  mark(LA_PHASE_TRANSPOSE, 0);
-  K<T>::transpose(m, n, A, lda, Tm, ldt, s);  // T(c, r) = A(r, c)
-  int leafno = 0;
+  auto endm = [&](PhaseTimer& pt, std::vector<double>& fl) { mark(pt, fl, -1, 0); };
   auto Tp = [&](long r, long col) { return Tm + r + col * ldt; };
 
-  for (int j = 0; j < mn; j += nb) {
-    const int jb = std::min(nb, mn - j);
-    // ---- panel: A(j:m, j:j+jb) = T rows j..j+jb, T columns j..m
+  launch_fill_u32(tags, panel_tag_count(), 0u, s);
+  launch_fill_u32(rankctr, nrank, 0u, s);
+  launch_fill_i32(err, 1, 0, s);
+  launch_fill_i32(info, 1, 0, s);
+  mark(pa, fla, LA_PHASE_TRANSPOSE, 0);
+  K<T>::transpose(m, n, A, lda, Tm, ldt, s);  // T(c, r) = A(r, c)
+  endm(pa, fla);
+
+  int leafno = 0;
+  // Panel j: A(j:m, j:j+jb) = T rows j..j+jb, T columns j..m; the DGETRF2
+  // recursion flattened to leaves, with in-panel interchanges, DTRSM and
+  // DGEMM between leaves (buffers of slot 1, stream st).
+  auto factor_panel = [&](int j, int jb, hipStream_t st, PhaseTimer& pt, std::vector<double>& fl,
+                          bool xcd0) -> int {
+    T* X = Xs[1];
+    T* W = Ws[1];
+    int* pairs = Ps[1];
+    int* npairs = pairs + 2 * swp_max_pairs();
     for (int jj = j; jj < j + jb;) {
       const int h = m - jj;
       const int w = std::min({tune_leaf(), panel_max_width(h), j + jb - jj});
-      mark(LA_PHASE_PANEL, 0);
-      K<T>::panel(Tm, ldt, jj, w, m, ipiv, info, ctr + leafno++, xchg, err, s);
-      mark(LA_PHASE_LASWP, 0);
-      if (jj > j) K<T>::laswp_cols(Tm, ldt, j, jj, jj, jj + w, ipiv, s);
+      mark(pt, fl, LA_PHASE_PANEL, 0);
+      ++leafno;
+      int r = K<T>::panel(Tm, ldt, jj, w, m, ipiv, info, 64u * (unsigned)leafno, xchg, tags, err,
+                          rankctr + (leafno % (int)nrank), xcd0, st);
+      endm(pt, fl);
+      if (r) return r;
       const int rest = j + jb - jj - w;
+      if (jj > j || rest > 0) {
+        mark(pt, fl, LA_PHASE_LASWP, 0);
+        launch_swaps_to_pairs(jj, jj + w, m, ipiv, pairs, npairs, gmap, st);
+        K<T>::apply_cols(Tm, ldt, j, jj, jj + w, j + jb, pairs, npairs, st);
+        endm(pt, fl);
+      }
       if (rest > 0) {
-        K<T>::laswp_cols(Tm, ldt, jj + w, j + jb, jj, jj + w, ipiv, s);
         // in-panel DTRSM: T(jj+w:j+jb, jj:jj+w) <- * inv(U'(w x w))
-        mark(LA_PHASE_TRSM, 2.0 * rest * w * w / 2.0);
-        K<T>::trtri_blocks(true, true, w, w, Tp(jj, jj), ldt, W, ldw, s);
-        K<T>::gemm('N', 'N', rest, w, w, T(1), Tp(jj + w, jj), ldt, W, ldw, T(0), X, ldx, s);
-        K<T>::copy(rest, w, X, ldx, Tp(jj + w, jj), ldt, s);
+        mark(pt, fl, LA_PHASE_TRSM, 2.0 * rest * w * w / 2.0);
+        K<T>::trtri_blocks(true, true, w, w, Tp(jj, jj), ldt, W, ldw, st);
+        K<T>::gemm('N', 'N', rest, w, w, T(1), Tp(jj + w, jj), ldt, W, ldw, T(0), X, ldx, st);
+        K<T>::copy(rest, w, X, ldx, Tp(jj + w, jj), ldt, st);
+        endm(pt, fl);
         // in-panel DGEMM: T(jj+w:j+jb, jj+w:m) -= T(jj+w:j+jb, jj:jj+w) * T(jj:jj+w, jj+w:m)
         if (m - jj - w > 0) {
-          mark(LA_PHASE_GEMM_PANEL, 2.0 * rest * (double)(m - jj - w) * w);
+          mark(pt, fl, LA_PHASE_GEMM_PANEL, 2.0 * rest * (double)(m - jj - w) * w);
           K<T>::gemm('N', 'N', rest, m - jj - w, w, T(-1), Tp(jj + w, jj), ldt, Tp(jj, jj + w), ldt, T(1),
-                     Tp(jj + w, jj + w), ldt, s);
+                     Tp(jj + w, jj + w), ldt, st);
+          endm(pt, fl);
         }
       }
       jj += w;
     }
+    return 0;
+  };
+
+  // panel 0 on the main stream
+  rc = factor_panel(0, std::min(nb, mn), s, pa, fla, false);
+  for (int j = 0; j < mn && rc == 0; j += nb) {
+    const int jb = std::min(nb, mn - j);
+    if (j > 0) LA_TRY(hipStreamWaitEvent(s, c->ev[0], 0));  // panel j done (panel stream)
+    T* X = Xs[0];
+    T* W = Ws[0];
+    int* pairs = Ps[0];
+    int* npairs = pairs + 2 * swp_max_pairs();
     // ---- interchanges outside the panel (SRC/dgetrf.f:193,199-200)
-    mark(LA_PHASE_LASWP, 0);
-    if (j > 0) K<T>::laswp_cols(Tm, ldt, 0, j, j, j + jb, ipiv, s);
-    if (j + jb < n) {
-      K<T>::laswp_cols(Tm, ldt, j + jb, n, j, j + jb, ipiv, s);
-      const int n2 = n - j - jb;
-      // ---- DTRSM: U12^T = A12^T * inv(L11^T)
-      mark(LA_PHASE_TRSM, 2.0 * n2 * jb * (double)jb / 2.0);
-      K<T>::trtri_blocks(true, true, jb, jb, Tp(j, j), ldt, W, ldw, s);
-      K<T>::gemm('N', 'N', n2, jb, jb, T(1), Tp(j + jb, j), ldt, W, ldw, T(0), X, ldx, s);
-      K<T>::copy(n2, jb, X, ldx, Tp(j + jb, j), ldt, s);
-      // ---- DGEMM: A22 -= A21*U12, i.e. T22 -= T12 * T21
-      if (j + jb < m) {
-        mark(LA_PHASE_GEMM, 2.0 * n2 * (double)(m - j - jb) * jb);
-        K<T>::gemm('N', 'N', n2, m - j - jb, jb, T(-1), X, ldx, Tp(j, j + jb), ldt, T(1), Tp(j + jb, j + jb),
-                   ldt, s);
-      }
+    mark(pa, fla, LA_PHASE_LASWP, 0);
+    if (j > 0 || j + jb < n) {
+      launch_swaps_to_pairs(j, j + jb, m, ipiv, pairs, npairs, gmap, s);
+      K<T>::apply_cols(Tm, ldt, 0, j, j + jb, n, pairs, npairs, s);
+    }
+    endm(pa, fla);
+    if (j + jb >= n) continue;
+    const int n2 = n - j - jb;
+    // ---- DTRSM: U12^T = A12^T * inv(L11^T)  (SRC/dgetrf.f:204-207)
+    mark(pa, fla, LA_PHASE_TRSM, 2.0 * n2 * jb * (double)jb / 2.0);
+    K<T>::trtri_blocks(true, true, jb, jb, Tp(j, j), ldt, W, ldw, s);
+    K<T>::gemm('N', 'N', n2, jb, jb, T(1), Tp(j + jb, j), ldt, W, ldw, T(0), X, ldx, s);
+    K<T>::copy(n2, jb, X, ldx, Tp(j + jb, j), ldt, s);
+    endm(pa, fla);
+    if (j + jb >= m) continue;
+    // ---- DGEMM: A22 -= A21*U12, i.e. T22 -= T12 * T21 (SRC/dgetrf.f:212-216),
+    //      the next panel's columns first
+    const int jn = j + jb;
+    const int jbn = jn < mn ? std::min(nb, mn - jn) : 0;
+    const int mrows = m - jn;
+    // the next panel takes XCD 0 for itself when it runs multi-workgroup
+    const bool split = xcdsplit && jbn > 0 && panel_groups(m - jn) > 0 && panel_groups(m - jn) <= 32;
+    if (jbn > 0) {
+      mark(pa, fla, LA_PHASE_GEMM, 2.0 * jbn * (double)mrows * jb);
+      K<T>::update(jbn, mrows, jb, X, ldx, Tp(j, jn), ldt, Tp(jn, jn), ldt, false, s);
+      endm(pa, fla);
+      LA_TRY(hipEventRecord(c->ev[1], s));
+      LA_TRY(hipStreamWaitEvent(sp, c->ev[1], 0));
+      rc = factor_panel(jn, jbn, sp, pp, flp, split);
+      if (rc) break;
+      LA_TRY(hipEventRecord(c->ev[0], sp));
+    }
+    if (n2 - jbn > 0) {
+      mark(pa, fla, LA_PHASE_GEMM, 2.0 * (n2 - jbn) * (double)mrows * jb);
+      K<T>::update(n2 - jbn, mrows, jb, X + jbn, ldx, Tp(j, jn), ldt, Tp(jn + jbn, jn), ldt, split, s);
+      endm(pa, fla);
     }
   }
-  mark(LA_PHASE_TRANSPOSE, 0);
+  if (rc) {
+    // drain the panel stream before giving the workspace back
+    hipStreamSynchronize(sp);
+    hipStreamSynchronize(s);
+    ws_release(c, s_user);
+    return rc;
+  }
+  mark(pa, fla, LA_PHASE_TRANSPOSE, 0);
   K<T>::transpose(n, m, Tm, ldt, A, lda, s);
-  mark(-1, 0);
+  endm(pa, fla);
   launch_fold_err(err, info, s);
-  rc = ws_release(c, s);
-  phase_collect(pt, fl);
+  LA_TRY(hipEventRecord(c->ev[3], s));
+  LA_TRY(hipStreamWaitEvent(s_user, c->ev[3], 0));
+  rc = ws_release(c, s_user);
+  phase_collect(pa, fla);
+  phase_collect(pp, flp);
   return rc;
 }
 
@@ -503,9 +660,12 @@ DevCtx* ctx_for_current() { return get_ctx(); }
 hipStream_t ctx_stream(DevCtx* c) { return c->stream; }
 int io_buffer(DevCtx* c, size_t bytes, char** out) { return io_acquire(c, bytes, out); }
 std::mutex& io_mutex(DevCtx* c) { return c->io_mu; }
+int mp_buffer(DevCtx* c, size_t bytes, char** out) { return mp_acquire(c, bytes, out); }
+std::mutex& mp_mutex(DevCtx* c) { return c->mp_mu; }
 void phase_enable(int on) { g_phase_on = on; }
 void phase_read(double* ms, long* calls, double* flops) {
   std::lock_guard<std::mutex> g(g_phase_mu);
+  phase_resolve_locked();
   for (int i = 0; i < LA_PHASE_COUNT; ++i) {
     if (ms) ms[i] = g_phase_ms[i];
     if (calls) calls[i] = g_phase_calls[i];
@@ -514,6 +674,7 @@ void phase_read(double* ms, long* calls, double* flops) {
 }
 void phase_reset() {
   std::lock_guard<std::mutex> g(g_phase_mu);
+  phase_resolve_locked();
   for (int i = 0; i < LA_PHASE_COUNT; ++i) { g_phase_ms[i] = 0; g_phase_calls[i] = 0; g_phase_flops[i] = 0; }
 }
 

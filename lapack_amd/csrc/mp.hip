@@ -1,0 +1,156 @@
+// mp.hip -- element-wise and reduction kernels of the mixed-precision solver:
+// DLAG2S / SLAG2D (SRC/dlag2s.f:105-151, SRC/slag2d.f:101-133), the norms the
+// convergence test uses (DLANGE 'I', SRC/dlange.f:173-186; the IDAMAX column
+// maxima of SRC/dsgesv.f:337-342), DAXPY, and the power-of-two
+// equilibration (DGEEQUB, SRC/dgeequb.f:213-311; DLAQGE, SRC/dlaqge.f:183-226).
+#include "la_common.h"
+
+namespace la {
+namespace {
+
+inline int grid_for(long work) {
+  long g = (work + 255) / 256;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+// SA = REAL(A); *overflow = 1 when |a| > SLAMCH('O') for any entry.
+__global__ __launch_bounds__(256) void cast_d2s_kernel(int m, int n, const double* __restrict__ A, long lda,
+                                                       float* __restrict__ S, long lds, int* __restrict__ ovf) {
+  const long tot = (long)m * n;
+  bool bad = false;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
+    const long i = e % m, j = e / m;
+    const double v = A[i + j * lda];
+    if (v < -3.4028234663852886e38 || v > 3.4028234663852886e38) bad = true;
+    S[i + j * lds] = (float)v;
+  }
+  if (bad) atomicOr(ovf, 1);
+}
+
+__global__ __launch_bounds__(256) void cast_s2d_kernel(int m, int n, const float* __restrict__ S, long lds,
+                                                       double* __restrict__ A, long lda) {
+  const long tot = (long)m * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
+    const long i = e % m, j = e / m;
+    A[i + j * lda] = (double)S[i + j * lds];
+  }
+}
+
+// y(:,j) += x(:,j) (alpha = 1, SRC/dsgesv.f:381-383)
+__global__ __launch_bounds__(256) void axpy_kernel(int m, int n, const double* __restrict__ X, long ldx,
+                                                   double* __restrict__ Y, long ldy) {
+  const long tot = (long)m * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
+    const long i = e % m, j = e / m;
+    Y[i + j * ldy] = Y[i + j * ldy] + X[i + j * ldx];
+  }
+}
+
+// out[j] = |A(IDAMAX(m, A(1,j), 1), j)| as the bit pattern of a non-negative
+// double (atomicMax on the bits orders like the values).  IDAMAX's strict '>'
+// (BLAS/SRC/idamax.f:112-118) skips a NaN anywhere but in row 1, and a NaN
+// in row 1 is the result; a positive NaN's bits exceed every number's, so
+// row 1's NaN wins the atomicMax and all other NaNs are dropped here.
+__global__ __launch_bounds__(256) void col_absmax_kernel(int m, int n, const double* __restrict__ A, long lda,
+                                                         unsigned long long* __restrict__ out) {
+  const int j = blockIdx.y;
+  unsigned long long mx = 0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < m; i += (long)gridDim.x * 256) {
+    const double a = fabs(A[i + (long)j * lda]);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(a);
+    if ((a == a || i == 0) && b > mx) mx = b;
+  }
+  __shared__ unsigned long long s[256];
+  s[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o && s[threadIdx.x + o] > s[threadIdx.x]) s[threadIdx.x] = s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(&out[j], s[0]);
+}
+
+// rowsum[i] = sum_j |A(i,j)| (the WORK vector of DLANGE 'I'), j-order per row.
+__global__ __launch_bounds__(256) void row_abssum_kernel(int m, int n, const double* __restrict__ A, long lda,
+                                                         double* __restrict__ rowsum) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int j = 0; j < n; ++j) s = s + fabs(A[i + (long)j * lda]);
+  rowsum[i] = s;
+}
+
+// Row maxima r(i) = max_j |A(i,j)| (DGEEQU(B) first pass).
+__global__ __launch_bounds__(256) void row_absmax_kernel(int m, int n, const double* __restrict__ A, long lda,
+                                                         double* __restrict__ r) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  double mx = 0.0;
+  for (int j = 0; j < n; ++j) mx = fmax(mx, fabs(A[i + (long)j * lda]));
+  r[i] = mx;
+}
+
+// c(j) = max_i |A(i,j)| * r(i) (DGEEQU(B) second pass), one workgroup per column.
+__global__ __launch_bounds__(256) void col_absmax_scaled_kernel(int m, int n, const double* __restrict__ A, long lda,
+                                                                const double* __restrict__ r, double* __restrict__ c) {
+  const int j = blockIdx.x;
+  double mx = 0.0;
+  for (long i = threadIdx.x; i < m; i += 256) mx = fmax(mx, fabs(A[i + (long)j * lda]) * r[i]);
+  __shared__ double s[256];
+  s[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) c[j] = s[0];
+}
+
+// A(i,j) <- c(j)*r(i)*A(i,j) (or one factor): the scaling forms of DLAQGE.
+__global__ __launch_bounds__(256) void scale_rc_kernel(int m, int n, double* __restrict__ A, long lda,
+                                                       const double* __restrict__ r, const double* __restrict__ c,
+                                                       int mode) {
+  const long tot = (long)m * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
+    const long i = e % m, j = e / m;
+    double& a = A[i + j * lda];
+    if (mode == 1) a = r[i] * a;                 // EQUED = 'R'
+    else if (mode == 2) a = c[j] * a;            // 'C'
+    else if (mode == 3) a = c[j] * r[i] * a;     // 'B'
+  }
+}
+
+}  // namespace
+
+void launch_cast_d2s(int m, int n, const double* A, long lda, float* S, long lds, int* ovf, hipStream_t s) {
+  if (m > 0 && n > 0) cast_d2s_kernel<<<grid_for((long)m * n), 256, 0, s>>>(m, n, A, lda, S, lds, ovf);
+}
+void launch_cast_s2d(int m, int n, const float* S, long lds, double* A, long lda, hipStream_t s) {
+  if (m > 0 && n > 0) cast_s2d_kernel<<<grid_for((long)m * n), 256, 0, s>>>(m, n, S, lds, A, lda);
+}
+void launch_axpy(int m, int n, const double* X, long ldx, double* Y, long ldy, hipStream_t s) {
+  if (m > 0 && n > 0) axpy_kernel<<<grid_for((long)m * n), 256, 0, s>>>(m, n, X, ldx, Y, ldy);
+}
+void launch_col_absmax(int m, int n, const double* A, long lda, unsigned long long* out, hipStream_t s) {
+  if (m <= 0 || n <= 0) return;
+  launch_fill_i32(reinterpret_cast<int*>(out), (size_t)2 * n, 0, s);  // a kernel, not a memset (graph-safe)
+  int gx = (m + 255) / 256;
+  if (gx > 64) gx = 64;
+  col_absmax_kernel<<<dim3(gx, n), 256, 0, s>>>(m, n, A, lda, out);
+}
+void launch_row_abssum(int m, int n, const double* A, long lda, double* rowsum, hipStream_t s) {
+  if (m > 0) row_abssum_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, n, A, lda, rowsum);
+}
+void launch_row_absmax(int m, int n, const double* A, long lda, double* r, hipStream_t s) {
+  if (m > 0) row_absmax_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, n, A, lda, r);
+}
+void launch_col_absmax_scaled(int m, int n, const double* A, long lda, const double* r, double* c, hipStream_t s) {
+  if (n > 0) col_absmax_scaled_kernel<<<n, 256, 0, s>>>(m, n, A, lda, r, c);
+}
+void launch_scale_rc(int m, int n, double* A, long lda, const double* r, const double* c, int mode, hipStream_t s) {
+  if (m > 0 && n > 0 && mode > 0) scale_rc_kernel<<<grid_for((long)m * n), 256, 0, s>>>(m, n, A, lda, r, c, mode);
+}
+
+}  // namespace la

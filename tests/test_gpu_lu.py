@@ -1,6 +1,6 @@
 """GPU parity of the plain FP64/FP32 LU path against the oracle.
 
-Bars (SURVEY section 4/8c, the reference's own test ratios):
+Bars (the reference's own test ratios):
   * IPIV identical to the reference DGETRF (integer output).  A divergence is
     accepted only at a documented near-tie: at the first differing step k the
     reference's second-best candidate has |L(i,k)| >= 1 - 1e-9.
