@@ -115,6 +115,51 @@ int lapack_amd_dgemm_async(char transa, char transb, int m, int n, int k, double
 int lapack_amd_sgemm_async(char transa, char transb, int m, int n, int k, float alpha, const float* d_a,
                            int lda, const float* d_b, int ldb, float beta, float* d_c, int ldc, void* stream);
 
+/* Adjoint of the solve X = A^-1 B (reverse-mode derivative), from the saved
+ * DGETRF factors d_lu/d_ipiv and the solution d_x, for the loss gradient
+ * d_xbar:  Bbar = A^-T Xbar  -- DGETRS('T') on the saved factors, no
+ * refactorization (SRC/dgetrs.f:199-217) --  and Abar = -Bbar X^T, a
+ * rank-NRHS DGEMM('N','T') (BLAS/SRC/dgemm.f), added to d_abar when
+ * accumulate != 0.  d_bbar may equal d_xbar (in place).  The reference has
+ * no single routine for it; it is the composition of the two calls above.
+ * Returns 0 or -(parameter number) (N 1, NRHS 2, LDA 4, LDX 7, LDXBAR 9,
+ * LDBBAR 11, LDABAR 13). */
+int lapack_amd_dgesv_adjoint_async(int n, int nrhs, const double* d_lu, int lda, const int* d_ipiv,
+                                   const double* d_x, int ldx, const double* d_xbar, int ldxbar, double* d_bbar,
+                                   int ldbbar, double* d_abar, int ldabar, int accumulate, void* stream);
+
+/* ============================ 4. batched LU =============================== */
+
+/* Thousands of independent systems of uneven order n_i (any mix of orders
+ * up to the LDS limit: n <= 2500 in FP64, 5000 in FP32), each a
+ * column-major matrix of its own with its own LDA, addressed through device
+ * arrays of pointers.  The reference has no batched routine; each matrix
+ * gets exactly DGETRF (SRC/dgetrf.f:105) / DGETRS('N') (SRC/dgetrs.f:118)
+ * semantics: IPIV 1-based per matrix, INFO per matrix.
+ *
+ * A plan fixes the orders (host array n[batch]) and the precision ('d' or
+ * 's'), orders the matrices by decreasing n for the persistent launch, and
+ * -- with autotune != 0 -- times the kernel configurations (panel width NB,
+ * workgroup size) on synthetic matrices of the plan's sizes and keeps the
+ * fastest.  Creating a plan allocates; the *_batched_async launches do
+ * not, and may be captured in a hipGraph and replayed. */
+typedef struct lapack_amd_batch_plan lapack_amd_batch_plan;
+int lapack_amd_batch_plan_create(lapack_amd_batch_plan** plan, int batch, const int* n, char precision,
+                                 int autotune);
+void lapack_amd_batch_plan_destroy(lapack_amd_batch_plan* plan);
+int lapack_amd_batch_plan_config(const lapack_amd_batch_plan* plan, int* nb, int* nthreads, double* tuned_ms);
+int lapack_amd_batch_plan_set_config(lapack_amd_batch_plan* plan, int nb, int nthreads);
+int lapack_amd_dgetrf_batched_async(const lapack_amd_batch_plan* plan, double* const* d_a, const int* d_lda,
+                                    int* const* d_ipiv, int* d_info, void* stream);
+int lapack_amd_sgetrf_batched_async(const lapack_amd_batch_plan* plan, float* const* d_a, const int* d_lda,
+                                    int* const* d_ipiv, int* d_info, void* stream);
+int lapack_amd_dgetrs_batched_async(const lapack_amd_batch_plan* plan, const int* d_nrhs, double* const* d_a,
+                                    const int* d_lda, int* const* d_ipiv, double* const* d_b, const int* d_ldb,
+                                    void* stream);
+int lapack_amd_sgetrs_batched_async(const lapack_amd_batch_plan* plan, const int* d_nrhs, float* const* d_a,
+                                    const int* d_lda, int* const* d_ipiv, float* const* d_b, const int* d_ldb,
+                                    void* stream);
+
 /* ============================ tuning and tools ============================ */
 
 /* Block size NB of the outer loop (the ILAENV(1,'DGETRF') value, reference

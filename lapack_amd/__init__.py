@@ -68,6 +68,15 @@ def _declare(L):
         "lapack_amd_dlaswp_async": (i, [i, vp, i, i, i, vp, i, vp]),
         "lapack_amd_dgemm_async": (i, [ch, ch, i, i, i, d, vp, i, vp, i, d, vp, i, vp]),
         "lapack_amd_sgemm_async": (i, [ch, ch, i, i, i, f, vp, i, vp, i, f, vp, i, vp]),
+        "lapack_amd_dgesv_adjoint_async": (i, [i, i, vp, i, vp, vp, i, vp, i, vp, i, vp, i, i, vp]),
+        "lapack_amd_batch_plan_create": (i, [ctypes.POINTER(vp), i, vp, ch, i]),
+        "lapack_amd_batch_plan_destroy": (None, [vp]),
+        "lapack_amd_batch_plan_config": (i, [vp, _c_int_p, _c_int_p, _c_dbl_p]),
+        "lapack_amd_batch_plan_set_config": (i, [vp, i, i]),
+        "lapack_amd_dgetrf_batched_async": (i, [vp, vp, vp, vp, vp, vp]),
+        "lapack_amd_sgetrf_batched_async": (i, [vp, vp, vp, vp, vp, vp]),
+        "lapack_amd_dgetrs_batched_async": (i, [vp, vp, vp, vp, vp, vp, vp, vp]),
+        "lapack_amd_sgetrs_batched_async": (i, [vp, vp, vp, vp, vp, vp, vp, vp]),
         "lapack_amd_set_tuning": (None, [i, i]),
         "lapack_amd_get_tuning": (None, [_c_int_p, _c_int_p]),
         "lapack_amd_xerbla_record": (None, [i]),
@@ -189,6 +198,63 @@ def dgemm_async(transa, transb, m, n, k, alpha, d_a, lda, d_b, ldb, beta, d_c, l
 def sgemm_async(transa, transb, m, n, k, alpha, d_a, lda, d_b, ldb, beta, d_c, ldc, stream=None):
     return lib().lapack_amd_sgemm_async(_c(transa), _c(transb), m, n, k, float(alpha), _ptr(d_a), lda, _ptr(d_b),
                                         ldb, float(beta), _ptr(d_c), ldc, _stream(stream))
+
+
+def dgesv_adjoint_async(n, nrhs, d_lu, lda, d_ipiv, d_x, ldx, d_xbar, ldxbar, d_bbar, ldbbar, d_abar, ldabar,
+                        accumulate=False, stream=None):
+    """Bbar = A^-T Xbar (DGETRS('T') on the saved LU) and Abar = -Bbar X^T."""
+    return lib().lapack_amd_dgesv_adjoint_async(n, nrhs, _ptr(d_lu), lda, _ptr(d_ipiv), _ptr(d_x), ldx, _ptr(d_xbar),
+                                                ldxbar, _ptr(d_bbar), ldbbar, _ptr(d_abar), ldabar,
+                                                1 if accumulate else 0, _stream(stream))
+
+
+# ---- batched LU ----------------------------------------------------------------
+
+class BatchPlan:
+    """A batch of independent systems of uneven order (the library's plan):
+    orders n_list, precision 'd' or 's'; autotune times the kernel
+    configurations on synthetic matrices of these orders and keeps the
+    fastest.  Launches take device arrays: pointers (int64 tensors holding
+    device addresses), leading dimensions and right-hand-side counts (int32)."""
+
+    def __init__(self, n_list, precision="d", autotune=True):
+        self.precision = precision
+        self.batch = len(n_list)
+        arr = (ctypes.c_int * max(1, self.batch))(*[int(v) for v in n_list])
+        h = ctypes.c_void_p()
+        rc = lib().lapack_amd_batch_plan_create(ctypes.byref(h), self.batch, ctypes.cast(arr, ctypes.c_void_p),
+                                                 _c(precision), 1 if autotune else 0)
+        if rc != 0:
+            raise RuntimeError(f"lapack_amd_batch_plan_create returned {rc}")
+        self._h = h
+
+    def config(self):
+        nb, nt, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        lib().lapack_amd_batch_plan_config(self._h, ctypes.byref(nb), ctypes.byref(nt), ctypes.byref(ms))
+        return nb.value, nt.value, ms.value
+
+    def set_config(self, nb, nthreads):
+        return lib().lapack_amd_batch_plan_set_config(self._h, nb, nthreads)
+
+    def getrf(self, d_aptr, d_lda, d_ipivptr, d_info, stream=None):
+        fn = lib().lapack_amd_dgetrf_batched_async if self.precision == "d" else lib().lapack_amd_sgetrf_batched_async
+        return fn(self._h, _ptr(d_aptr), _ptr(d_lda), _ptr(d_ipivptr), _ptr(d_info), _stream(stream))
+
+    def getrs(self, d_nrhs, d_aptr, d_lda, d_ipivptr, d_bptr, d_ldb, stream=None):
+        fn = lib().lapack_amd_dgetrs_batched_async if self.precision == "d" else lib().lapack_amd_sgetrs_batched_async
+        return fn(self._h, _ptr(d_nrhs), _ptr(d_aptr), _ptr(d_lda), _ptr(d_ipivptr), _ptr(d_bptr), _ptr(d_ldb),
+                  _stream(stream))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lapack_amd_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---- tuning, error hook, profiling, generator -------------------------------

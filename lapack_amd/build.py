@@ -22,7 +22,10 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wno-unused-result"]
+# .cpp sources hold host code only; hipcc still runs a device pass over
+# them, so they get the same target and code-generation flags.
+HOST_FLAGS = [f"--offload-arch={ARCH}", "-O2", "-fPIC", "-std=c++17", "-Wno-unused-result",
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def hipcc():

@@ -281,7 +281,7 @@ void launch_transpose_d2s(int rows, int cols, const double* in, long ld_in, floa
   launch_transpose_t<double, float, true>(rows, cols, in, ld_in, out, ld_out, overflow, s);
 }
 
-static int lds_limit() {
+int lds_limit() {
   static int lds_max = -1;
   if (lds_max < 0) {
     int dev = 0, v = 65536;

@@ -1,0 +1,494 @@
+// batched.hip -- many independent LU factorizations and solves of uneven
+// order in one persistent launch: one workgroup per matrix at a time.
+//
+// The reference has no batched API; its semantics are a loop of DGETRF
+// (SRC/dgetrf.f:105-225) and DGETRS (SRC/dgetrs.f:181-218) over the
+// matrices, and that is what each workgroup computes for the matrix it takes.
+//
+// Scheduling: the launch has as many workgroups as fit on the device at once
+// (a persistent grid).  Each workgroup takes the next matrix from a work
+// counter over an order sorted by decreasing n (largest first, which keeps
+// the tail of the launch short), so the grid drains when the counter passes
+// the batch size; every loop is bounded by the matrix order.
+//
+// DGETRF per matrix, right-looking and blocked with panel width NB:
+//   1. the panel A(k0:n, k0:k0+w) is loaded into LDS and factored there
+//      column by column: IDAMAX (BLAS/SRC/idamax.f:93-122) by wavefront
+//      reductions over 64-bit keys that order like its rule (first max,
+//      strict '>', a NaN at the head of the column wins, others never), the
+//      interchange, the scaling by the reciprocal or by division below sfmin
+//      (SRC/dgetrf2.f:196-210) and the rank-1 update of the panel;
+//   2. the panel is written back and its interchanges are applied to the
+//      columns left of it (SRC/dgetrf.f:193);
+//   3. the columns right of it, BC at a time: interchanges (dgetrf.f:199-200),
+//      the unit lower solve U12 = L11^-1 A12 (dgetrf.f:204-207, the loop order
+//      of BLAS/SRC/dtrsm.f:289-300 with its zero test), staged in LDS, then
+//      the update A22 -= A21*U12 (dgetrf.f:212-216) in 4x4 register tiles.
+// DGETRS('N') per matrix: the interchanges on B, then blocked forward and
+// backward substitution -- one wavefront solves each 64-row diagonal block
+// (dtrsm.f:277-300 order, with the reference's zero tests), the workgroup
+// updates the rows beyond it.
+#include "la_common.h"
+
+namespace la {
+namespace {
+
+typedef unsigned long long u64;
+
+template <class T> struct BSfmin;
+template <> struct BSfmin<double> { static constexpr double v = 0x1p-1022; };
+template <> struct BSfmin<float> { static constexpr float v = 0x1p-126f; };
+
+template <class T>
+__device__ __forceinline__ u64 bkey(T x, bool head) {
+  const double a = fabs((double)x);
+  if (a != a) return head ? 0x7FF0000000000001ull : 1ull;
+  return (u64)__double_as_longlong(a) + 1ull;
+}
+
+__device__ __forceinline__ void wave_argmax(u64& key, int& row) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const u64 k2 = __shfl_xor(key, off);
+    const int r2 = __shfl_xor(row, off);
+    if (k2 > key || (k2 == key && r2 < row)) { key = k2; row = r2; }
+  }
+}
+
+constexpr int TM = 4, TN = 4, TX = 16;
+
+// Barrier after global-memory writes that other waves of the workgroup read
+// next: wait for this wave's stores to complete, then synchronize.
+__device__ __forceinline__ void barrier_vm() {
+  wait_vm();
+  __syncthreads();
+}
+
+template <class T, int NB, int NT>
+__global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int* __restrict__ nv,
+                                                           T* const* __restrict__ av, const int* __restrict__ ldav,
+                                                           int* const* __restrict__ ipv, int* __restrict__ infov,
+                                                           const int* __restrict__ order, unsigned* counter, int ldp) {
+  constexpr int TY = NT / TX;
+  constexpr int BC = TY * TN;  // columns per column block
+  constexpr int BR = TX * TM;  // rows per row block
+  constexpr int NW = NT / 64;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* P = reinterpret_cast<T*>(smem_raw);  // panel, P[c*ldp + r], r relative to k0
+  T* U = P + (size_t)NB * ldp;            // U12 column block, U[j*BC + c]
+  __shared__ u64 s_key[NW];
+  __shared__ int s_row[NW];
+  __shared__ int s_piv[NB];
+  __shared__ int s_item, s_info;
+  const int tid = threadIdx.x;
+  const T sfmin = BSfmin<T>::v;
+
+  for (int guard = 0; guard <= batch; ++guard) {
+    if (tid == 0) s_item = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int item = s_item;
+    if (item >= batch) break;
+    const int b = order[item];
+    const int n = nv[b];
+    T* A = av[b];
+    const long lda = ldav[b];
+    int* ipiv = ipv[b];
+    if (tid == 0) s_info = 0;
+    for (int k0 = 0; k0 < n; k0 += NB) {
+      const int w = min(NB, n - k0);
+      const int mr = n - k0;
+      for (int r = tid; r < mr; r += NT)
+        for (int c = 0; c < w; ++c) P[c * ldp + r] = A[(k0 + r) + (long)(k0 + c) * lda];
+      __syncthreads();
+      // ---- panel in LDS
+      for (int j = 0; j < w; ++j) {
+        u64 key = 0;
+        int row = 0x7fffffff;
+        for (int r = j + tid; r < mr; r += NT) {
+          const u64 kk = bkey(P[j * ldp + r], r == j);
+          if (kk > key) { key = kk; row = r; }
+        }
+        wave_argmax(key, row);
+        if ((tid & 63) == 0) { s_key[tid >> 6] = key; s_row[tid >> 6] = row; }
+        __syncthreads();
+        if (tid == 0) {
+          u64 bk = s_key[0];
+          int br = s_row[0];
+          for (int q = 1; q < NW; ++q)
+            if (s_key[q] > bk || (s_key[q] == bk && s_row[q] < br)) { bk = s_key[q]; br = s_row[q]; }
+          if (br >= mr) br = j;  // cannot happen (row j is always a candidate); keeps indexes in range
+          s_piv[j] = br;
+          ipiv[k0 + j] = k0 + br + 1;
+          if (P[j * ldp + br] == T(0) && s_info == 0) s_info = k0 + j + 1;
+        }
+        __syncthreads();
+        const int p = s_piv[j];
+        const T pv = P[j * ldp + p];
+        if (pv != T(0)) {
+          if (p != j)
+            for (int c = tid; c < w; c += NT) {
+              const T t = P[c * ldp + j];
+              P[c * ldp + j] = P[c * ldp + p];
+              P[c * ldp + p] = t;
+            }
+          __syncthreads();
+          if (fabs(pv) >= sfmin) {
+            const T rcp = T(1) / pv;
+            for (int r = j + 1 + tid; r < mr; r += NT) P[j * ldp + r] = rcp * P[j * ldp + r];
+          } else {
+            for (int r = j + 1 + tid; r < mr; r += NT) P[j * ldp + r] = P[j * ldp + r] / pv;
+          }
+        }
+        __syncthreads();
+        for (int r = j + 1 + tid; r < mr; r += NT) {
+          const T l = P[j * ldp + r];
+          for (int c = j + 1; c < w; ++c) P[c * ldp + r] = P[c * ldp + r] - l * P[c * ldp + j];
+        }
+        __syncthreads();
+      }
+      // ---- panel back to HBM; interchanges on the columns to its left
+      for (int r = tid; r < mr; r += NT)
+        for (int c = 0; c < w; ++c) A[(k0 + r) + (long)(k0 + c) * lda] = P[c * ldp + r];
+      for (int c = tid; c < k0; c += NT) {
+        T* col = A + (long)c * lda;
+        for (int j = 0; j < w; ++j) {
+          const int p = k0 + s_piv[j];
+          if (p != k0 + j) { const T t = col[k0 + j]; col[k0 + j] = col[p]; col[p] = t; }
+        }
+      }
+      // ---- columns to the right, BC at a time
+      const int c1 = k0 + w;
+      for (int cb = c1; cb < n; cb += BC) {
+        if (tid < BC) {
+          const int c = cb + tid;
+          if (c < n) {
+            T* col = A + (long)c * lda;
+            for (int j = 0; j < w; ++j) {
+              const int p = k0 + s_piv[j];
+              if (p != k0 + j) { const T t = col[k0 + j]; col[k0 + j] = col[p]; col[p] = t; }
+            }
+            T x[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) x[j] = (j < w) ? col[k0 + j] : T(0);
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+              if (j < w && x[j] != T(0)) {
+#pragma unroll
+                for (int i = j + 1; i < NB; ++i)
+                  if (i < w) x[i] = x[i] - x[j] * P[j * ldp + i];
+              }
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+              if (j < w) { col[k0 + j] = x[j]; U[j * BC + tid] = x[j]; }
+          } else {
+#pragma unroll
+            for (int j = 0; j < NB; ++j) U[j * BC + tid] = T(0);
+          }
+        }
+        barrier_vm();  // the update reads rows this phase's interchanges wrote
+        const int tx = tid % TX, ty = tid / TX;
+        const int cc = ty * TN;  // tile column offset within the block
+        for (int rb = c1; rb < n; rb += BR) {
+          const int r0 = rb + tx * TM;  // absolute row
+          T acc[TM][TN];
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int q = 0; q < TN; ++q) acc[i][q] = T(0);
+          if (r0 < n) {
+            const int nrow = min(TM, n - r0);
+            for (int j = 0; j < w; ++j) {
+              T a[TM], u[TN];
+#pragma unroll
+              for (int i = 0; i < TM; ++i) a[i] = (i < nrow) ? P[j * ldp + (r0 - k0) + i] : T(0);
+#pragma unroll
+              for (int q = 0; q < TN; ++q) u[q] = U[j * BC + cc + q];
+#pragma unroll
+              for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int q = 0; q < TN; ++q) acc[i][q] = fma(a[i], u[q], acc[i][q]);
+            }
+#pragma unroll
+            for (int q = 0; q < TN; ++q) {
+              const int c = cb + cc + q;
+              if (c < n) {
+                T* col = A + (long)c * lda;
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+                  if (r0 + i < n) col[r0 + i] = col[r0 + i] - acc[i][q];
+              }
+            }
+          }
+        }
+        __syncthreads();
+      }
+      barrier_vm();  // the next panel loads what this one's update wrote
+    }
+    if (tid == 0) infov[b] = s_info;
+    __syncthreads();
+  }
+}
+
+// DGETRS('N') for one matrix per workgroup, NR right-hand sides at a time.
+template <class T, int NT, int NR>
+__global__ __launch_bounds__(NT) void getrs_batched_kernel(int batch, const int* __restrict__ nv,
+                                                           const int* __restrict__ nrhsv, T* const* __restrict__ av,
+                                                           const int* __restrict__ ldav, int* const* __restrict__ ipv,
+                                                           T* const* __restrict__ bv, const int* __restrict__ ldbv,
+                                                           const int* __restrict__ order, unsigned* counter, int ldx) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* X = reinterpret_cast<T*>(smem_raw);  // X[r + q*ldx]
+  __shared__ int s_item;
+  const int tid = threadIdx.x, lane = tid & 63;
+  for (int guard = 0; guard <= batch; ++guard) {
+    if (tid == 0) s_item = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int item = s_item;
+    if (item >= batch) break;
+    const int b = order[item];
+    const int n = nv[b], nrhs = nrhsv[b];
+    const T* A = av[b];
+    const long lda = ldav[b];
+    const int* ipiv = ipv[b];
+    T* B = bv[b];
+    const long ldb = ldbv[b];
+    for (int q0 = 0; q0 < nrhs; q0 += NR) {
+      const int nr = min(NR, nrhs - q0);
+      for (int q = 0; q < nr; ++q)
+        for (int r = tid; r < n; r += NT) X[r + q * ldx] = B[r + (long)(q0 + q) * ldb];
+      __syncthreads();
+      if (tid < nr) {  // interchanges, in order (SRC/dgetrs.f:187)
+        for (int k = 0; k < n; ++k) {
+          int p = ipiv[k] - 1;
+          if (p < 0 || p >= n) p = k;
+          if (p != k) { const T t = X[k + tid * ldx]; X[k + tid * ldx] = X[p + tid * ldx]; X[p + tid * ldx] = t; }
+        }
+      }
+      __syncthreads();
+      // L Y = X, unit lower (dgetrs.f:191)
+      for (int kb = 0; kb < n; kb += 64) {
+        const int bs = min(64, n - kb);
+        if (tid < 64) {
+          T xi[NR];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) xi[q] = (q < nr && lane < bs) ? X[kb + lane + q * ldx] : T(0);
+          for (int k = 0; k < bs; ++k) {
+            const T l = (lane > k && lane < bs) ? A[(kb + lane) + (long)(kb + k) * lda] : T(0);
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+              const T xk = __shfl(xi[q], k);
+              if (xk != T(0) && lane > k && lane < bs) xi[q] = xi[q] - xk * l;
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+            if (q < nr && lane < bs) X[kb + lane + q * ldx] = xi[q];
+        }
+        __syncthreads();
+        for (int i = kb + bs + tid; i < n; i += NT) {
+          T xi[NR];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) xi[q] = (q < nr) ? X[i + q * ldx] : T(0);
+          for (int k = 0; k < bs; ++k) {
+            const T l = A[i + (long)(kb + k) * lda];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+              const T xk = (q < nr) ? X[kb + k + q * ldx] : T(0);
+              if (xk != T(0)) xi[q] = xi[q] - xk * l;
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+            if (q < nr) X[i + q * ldx] = xi[q];
+        }
+        __syncthreads();
+      }
+      // U X = Y, non-unit upper (dgetrs.f:197)
+      for (int kb = ((n - 1) / 64) * 64; kb >= 0; kb -= 64) {
+        const int bs = min(64, n - kb);
+        if (tid < 64) {
+          T xi[NR];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) xi[q] = (q < nr && lane < bs) ? X[kb + lane + q * ldx] : T(0);
+          const T dii = (lane < bs) ? A[(kb + lane) + (long)(kb + lane) * lda] : T(1);
+          for (int k = bs - 1; k >= 0; --k) {
+            const T u = (lane < k) ? A[(kb + lane) + (long)(kb + k) * lda] : T(0);
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+              if (lane == k && xi[q] != T(0)) xi[q] = xi[q] / dii;
+              const T xk = __shfl(xi[q], k);
+              if (xk != T(0) && lane < k) xi[q] = xi[q] - xk * u;
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+            if (q < nr && lane < bs) X[kb + lane + q * ldx] = xi[q];
+        }
+        __syncthreads();
+        for (int i = tid; i < kb; i += NT) {
+          T xi[NR];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) xi[q] = (q < nr) ? X[i + q * ldx] : T(0);
+          for (int k = bs - 1; k >= 0; --k) {
+            const T u = A[i + (long)(kb + k) * lda];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+              const T xk = (q < nr) ? X[kb + k + q * ldx] : T(0);
+              if (xk != T(0)) xi[q] = xi[q] - xk * u;
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+            if (q < nr) X[i + q * ldx] = xi[q];
+        }
+        __syncthreads();
+      }
+      for (int q = 0; q < nr; ++q)
+        for (int r = tid; r < n; r += NT) B[r + (long)(q0 + q) * ldb] = X[r + q * ldx];
+      __syncthreads();
+    }
+  }
+}
+
+// uniform(-1,1) from a hash of the element index: synthetic input of the
+// autotuner's timing runs.
+template <class T>
+__global__ void fill_uniform_kernel(T* a, size_t count, unsigned long long seed) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    unsigned long long z = seed + i * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    a[i] = (T)((double)(z >> 11) * 0x1p-52 - 1.0);
+  }
+}
+
+template <class T, int NB, int NT>
+size_t getrf_smem(int ldp) {
+  return sizeof(T) * ((size_t)NB * ldp + (size_t)NB * (NT / TX) * TN);
+}
+
+// Workgroups of the persistent grid: all that fit on the device at once.
+template <class T, int NB, int NT>
+int grid_cfg(int batch, int nmax) {
+  const int ldp = (nmax + 3) & ~3;
+  const size_t sm = getrf_smem<T, NB, NT>(ldp);
+  if (sm + 1024 > (size_t)lds_limit()) return LA_ERR_LIMIT;  // static LDS of the kernel included
+  auto kern = getrf_batched_kernel<T, NB, NT>;
+  if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit()) != hipSuccess)
+    (void)hipGetLastError();
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NT, sm) != hipSuccess || occ < 1) {
+    (void)hipGetLastError();
+    occ = 1;
+  }
+  int dev = 0, ncu = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  return std::max(1, std::min(batch, ncu * occ));
+}
+
+template <class T, int NB, int NT>
+void launch_getrf_cfg(int grid, int batch, int nmax, const int* nv, T* const* av, const int* ldav, int* const* ipv,
+                      int* infov, const int* order, unsigned* counter, hipStream_t s) {
+  const int ldp = (nmax + 3) & ~3;  // 32-byte aligned columns for the tile loads
+  const size_t sm = getrf_smem<T, NB, NT>(ldp);
+  launch_fill_u32(counter, 1, 0u, s);
+  getrf_batched_kernel<T, NB, NT><<<grid, NT, sm, s>>>(batch, nv, av, ldav, ipv, infov, order, counter, ldp);
+}
+
+}  // namespace
+
+void launch_fill_uniform_d(double* a, size_t count, unsigned long long seed, hipStream_t s) {
+  if (count) fill_uniform_kernel<double><<<1024, 256, 0, s>>>(a, count, seed);
+}
+void launch_fill_uniform_s(float* a, size_t count, unsigned long long seed, hipStream_t s) {
+  if (count) fill_uniform_kernel<float><<<1024, 256, 0, s>>>(a, count, seed);
+}
+
+// Configurations the autotuner chooses from: (NB, threads).
+int batched_getrf_configs() { return 6; }
+void batched_getrf_config(int idx, int* nb, int* nt) {
+  static const int tab[6][2] = {{8, 256}, {16, 256}, {16, 512}, {32, 256}, {32, 512}, {8, 128}};
+  *nb = tab[idx][0];
+  *nt = tab[idx][1];
+}
+size_t batched_getrf_smem(int idx, int nmax, int elem) {
+  int nb, nt;
+  batched_getrf_config(idx, &nb, &nt);
+  // + 1 KiB for the kernel's static LDS (pivot keys, rows, indexes)
+  return (size_t)elem * ((size_t)nb * ((nmax + 3) & ~3) + (size_t)nb * (nt / TX) * TN) + 1024;
+}
+
+template <class T>
+int batched_getrf_grid(int cfg, int batch, int nmax) {
+  switch (cfg) {
+    case 0: return grid_cfg<T, 8, 256>(batch, nmax);
+    case 1: return grid_cfg<T, 16, 256>(batch, nmax);
+    case 2: return grid_cfg<T, 16, 512>(batch, nmax);
+    case 3: return grid_cfg<T, 32, 256>(batch, nmax);
+    case 4: return grid_cfg<T, 32, 512>(batch, nmax);
+    case 5: return grid_cfg<T, 8, 128>(batch, nmax);
+  }
+  return LA_ERR_LIMIT;
+}
+template int batched_getrf_grid<double>(int, int, int);
+template int batched_getrf_grid<float>(int, int, int);
+
+template <class T>
+int launch_getrf_batched(int cfg, int grid, int batch, int nmax, const int* nv, T* const* av, const int* ldav,
+                         int* const* ipv, int* infov, const int* order, unsigned* counter, hipStream_t s) {
+  switch (cfg) {
+    case 0: launch_getrf_cfg<T, 8, 256>(grid, batch, nmax, nv, av, ldav, ipv, infov, order, counter, s); return 0;
+    case 1: launch_getrf_cfg<T, 16, 256>(grid, batch, nmax, nv, av, ldav, ipv, infov, order, counter, s); return 0;
+    case 2: launch_getrf_cfg<T, 16, 512>(grid, batch, nmax, nv, av, ldav, ipv, infov, order, counter, s); return 0;
+    case 3: launch_getrf_cfg<T, 32, 256>(grid, batch, nmax, nv, av, ldav, ipv, infov, order, counter, s); return 0;
+    case 4: launch_getrf_cfg<T, 32, 512>(grid, batch, nmax, nv, av, ldav, ipv, infov, order, counter, s); return 0;
+    case 5: launch_getrf_cfg<T, 8, 128>(grid, batch, nmax, nv, av, ldav, ipv, infov, order, counter, s); return 0;
+  }
+  return LA_ERR_LIMIT;
+}
+template int launch_getrf_batched<double>(int, int, int, int, const int*, double* const*, const int*, int* const*,
+                                          int*, const int*, unsigned*, hipStream_t);
+template int launch_getrf_batched<float>(int, int, int, int, const int*, float* const*, const int*, int* const*,
+                                         int*, const int*, unsigned*, hipStream_t);
+
+template <class T>
+int batched_getrs_grid(int batch, int nmax) {
+  constexpr int NT = 256, NR = 4;
+  const size_t sm = sizeof(T) * (size_t)NR * std::max(nmax, 1);
+  if (sm > (size_t)lds_limit()) return LA_ERR_LIMIT;
+  auto kern = getrs_batched_kernel<T, NT, NR>;
+  if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit()) != hipSuccess)
+    (void)hipGetLastError();
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NT, sm) != hipSuccess || occ < 1) {
+    (void)hipGetLastError();
+    occ = 1;
+  }
+  int dev = 0, ncu = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  return std::max(1, std::min(batch, ncu * occ));
+}
+template int batched_getrs_grid<double>(int, int);
+template int batched_getrs_grid<float>(int, int);
+
+template <class T>
+int launch_getrs_batched(int grid, int batch, int nmax, const int* nv, const int* nrhsv, T* const* av,
+                         const int* ldav, int* const* ipv, T* const* bv, const int* ldbv, const int* order,
+                         unsigned* counter, hipStream_t s) {
+  constexpr int NT = 256, NR = 4;
+  const size_t sm = sizeof(T) * (size_t)NR * std::max(nmax, 1);
+  launch_fill_u32(counter, 1, 0u, s);
+  getrs_batched_kernel<T, NT, NR><<<grid, NT, sm, s>>>(batch, nv, nrhsv, av, ldav, ipv, bv, ldbv, order, counter,
+                                                      std::max(nmax, 1));
+  return 0;
+}
+template int launch_getrs_batched<double>(int, int, int, const int*, const int*, double* const*, const int*,
+                                          int* const*, double* const*, const int*, const int*, unsigned*, hipStream_t);
+template int launch_getrs_batched<float>(int, int, int, const int*, const int*, float* const*, const int*,
+                                         int* const*, float* const*, const int*, const int*, unsigned*, hipStream_t);
+
+}  // namespace la

@@ -503,6 +503,32 @@ int lapack_amd_sgemm_async(char ta, char tb, int m, int n, int k, float alpha, c
   return hipc(hipGetLastError());
 }
 
+// ---- adjoint of X = A^-1 B -----------------------------------------------------
+// Bbar = A^-T Xbar by DGETRS('T') on the saved factors (SRC/dgetrs.f:199-217),
+// then Abar = -Bbar X^T (+ Abar if accumulate) as a rank-NRHS product.
+int lapack_amd_dgesv_adjoint_async(int n, int nrhs, const double* d_lu, int lda, const int* d_ipiv,
+                                   const double* d_x, int ldx, const double* d_xbar, int ldxbar, double* d_bbar,
+                                   int ldbbar, double* d_abar, int ldabar, int accumulate, void* stream) {
+  if (n < 0) return -1;
+  if (nrhs < 0) return -2;
+  if (lda < std::max(1, n)) return -4;
+  if (ldx < std::max(1, n)) return -7;
+  if (ldxbar < std::max(1, n)) return -9;
+  if (ldbbar < std::max(1, n)) return -11;
+  if (ldabar < std::max(1, n)) return -13;
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  if (d_bbar != d_xbar) launch_copy_d(n, nrhs, d_xbar, ldxbar, d_bbar, ldbbar, s);
+  int rc = getrs_device<double>('T', n, nrhs, d_lu, lda, d_ipiv, d_bbar, ldbbar, s);
+  if (rc) return rc;
+  const double beta = accumulate ? 1.0 : 0.0;
+  if (nrhs <= rankk_max())
+    launch_rankk_nt_d(n, n, nrhs, -1.0, d_bbar, ldbbar, d_x, ldx, beta, d_abar, ldabar, s);
+  else
+    launch_dgemm('N', 'T', n, n, nrhs, -1.0, d_bbar, ldbbar, d_x, ldx, beta, d_abar, ldabar, s);
+  return hipc(hipGetLastError());
+}
+
 // ---- tuning and tools --------------------------------------------------------
 void lapack_amd_set_tuning(int nb, int leaf) { set_tuning(nb, leaf); }
 void lapack_amd_get_tuning(int* nb, int* leaf) {

@@ -123,4 +123,33 @@ void launch_row_absmax(int m, int n, const double* A, long lda, double* r, hipSt
 void launch_col_absmax_scaled(int m, int n, const double* A, long lda, const double* r, double* c, hipStream_t s);
 void launch_scale_rc(int m, int n, double* A, long lda, const double* r, const double* c, int mode, hipStream_t s);
 
+// Largest LDS allocation of one workgroup on the current device (bytes).
+int lds_limit();
+
+// Batched LU (batched.hip): one persistent launch over matrices of uneven
+// order; order[] lists matrix indices by decreasing n, counter is one
+// device word the launcher zeroes (with a kernel) before the launch.
+void launch_fill_uniform_d(double* a, size_t count, unsigned long long seed, hipStream_t s);
+void launch_fill_uniform_s(float* a, size_t count, unsigned long long seed, hipStream_t s);
+int batched_getrf_configs();
+void batched_getrf_config(int idx, int* nb, int* nt);
+size_t batched_getrf_smem(int idx, int nmax, int elem);
+// grid: workgroups of the persistent launch (batched_*_grid, computed once
+// per plan so that the launches themselves make no runtime queries).
+template <class T> int batched_getrf_grid(int cfg, int batch, int nmax);
+template <class T> int batched_getrs_grid(int batch, int nmax);
+template <class T>
+int launch_getrf_batched(int cfg, int grid, int batch, int nmax, const int* nv, T* const* av, const int* ldav,
+                         int* const* ipv, int* infov, const int* order, unsigned* counter, hipStream_t s);
+template <class T>
+int launch_getrs_batched(int grid, int batch, int nmax, const int* nv, const int* nrhsv, T* const* av,
+                         const int* ldav, int* const* ipv, T* const* bv, const int* ldbv, const int* order,
+                         unsigned* counter, hipStream_t s);
+
+// Rank-k outer product C <- alpha*A*B^T + beta*C for k <= rankk_max()
+// (adjoint.hip), the Abar = -Bbar X^T step of the solve's adjoint.
+int rankk_max();
+void launch_rankk_nt_d(int m, int n, int k, double alpha, const double* A, long lda, const double* B, long ldb,
+                       double beta, double* C, long ldc, hipStream_t s);
+
 }  // namespace la

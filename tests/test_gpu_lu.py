@@ -236,3 +236,37 @@ def test_determinism(dev):
     lu2, p2, _ = gpu_getrf(a)
     assert np.array_equal(p1, p2)
     assert np.array_equal(lu1.view(np.int64), lu2.view(np.int64))
+
+
+def test_golden_ipiv_n1024(dev):
+    """GPU DGETRF on the golden input reproduces the reference's IPIV and
+    solves to the reference's x within the DGET02 bar."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "dgetrf_n1024_u.npz"))
+    n = 1024
+    a = la.matrix(2, 1988, n, n)
+    b = la.matrix(2, 1989, n, 1)
+    lu, ipiv, info = gpu_getrf(a)
+    assert info == 0 and np.array_equal(ipiv, g["ipiv"])
+    d_a, d_b = to_dev(lu), to_dev(b)
+    assert la.dgetrs("N", n, 1, d_a, n, to_dev(ipiv.astype(np.int32)).view(-1), d_b, n) == 0
+    x = from_dev(d_b, (n, 1))
+    assert oracle.dget02(a, x, b) < THRESH
+    assert np.max(np.abs(x[:, 0] - g["x"])) <= 1e-10 * np.max(np.abs(g["x"]))
+
+
+def test_golden_ipiv_n16384(dev):
+    """The benchmark's matrix (n = 16384, N(0,1), seed 16384): GPU IPIV equal to
+    the reference DGETRF's (tests/golden/dgetrf_n16384_normal_ipiv.npy)."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "dgetrf_n16384_normal_ipiv.npy")
+    if not os.path.exists(path):
+        pytest.skip("golden n=16384 IPIV not generated")
+    ref = np.load(path)
+    n = 16384
+    a = la.matrix(3, 16384, n, n)
+    d_a = to_dev(a)
+    del a
+    d_p = ivec(n)
+    assert la.dgetrf(n, n, d_a, n, d_p) == 0
+    assert np.array_equal(d_p.cpu().numpy(), ref)

@@ -3,6 +3,10 @@
 The reference's own driver test for it (TESTING/LIN/ddrvab.f:313-345) runs
 DSGESV and accepts the solution when DGET08's ratio
 ||B - A X||_inf / (||A||_inf ||X||_inf eps) stays below the threshold 30.
+DSGESV's stopping test (SRC/dsgesv.f:337-342) is that same ratio per column
+bounded by SQRT(N); the bar here is MAX(30, 2*SQRT(N)), doubled because the
+checker recomputes the residual with its own rounding (an error of the
+same order as the residual being measured).
 On top of that the GPU result is held to the oracle's:
   * ITER and INFO: the same branch of SRC/dsgesv.f is taken (overflow -2,
     FP32-singular -3, no convergence -31, else the iteration count).  The
@@ -26,6 +30,10 @@ pytestmark = pytest.mark.gpu
 THRESH = 30.0
 
 
+def dget08_bar(n):
+    return max(THRESH, 2.0 * np.sqrt(n))
+
+
 def gpu_dsgesv(a, b):
     n, nrhs = b.shape
     d_a = to_dev(a)
@@ -47,7 +55,7 @@ def test_dsgesv_refines_like_reference(dev, n, nrhs, seed):
     lu_s, _, _ = oracle.sgetrf(a)
     assert_ipiv(p, pr, lu_s.astype(np.float64))
     assert np.array_equal(a_after, a), "A must be unchanged when refinement succeeds"
-    assert oracle.dget08(a, x, b) < THRESH
+    assert oracle.dget08(a, x, b) < dget08_bar(n)
     # forward agreement with the oracle's refined solution
     assert np.max(np.abs(x - xr)) <= 1e3 * np.finfo(np.float64).eps * np.linalg.cond(a, np.inf) * np.max(np.abs(xr))
 

@@ -325,3 +325,21 @@ def test_planted_mistakes_are_caught():
     assert oracle.dget02(a, x, b) < 30
     assert oracle.dget02(a, x * (1 + 1e-10), b) > 30
     assert oracle.dget02(np.asfortranarray(a.T), x, b) > 30
+
+
+# ---------------------------------------------------------------- golden vectors
+
+def test_oracle_reproduces_reference_golden_n1024():
+    """tests/golden/dgetrf_n1024_u.npz holds reference DGETRF IPIV and DGETRS x
+    (flang build of the reference sources, tests/golden/make_golden.py) for
+    A uniform(-1,1) seed 1988, n = 1024, b seed 1989."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "dgetrf_n1024_u.npz"))
+    n = 1024
+    a = la.matrix(2, 1988, n, n)
+    b = la.matrix(2, 1989, n, 1)
+    lu, ipiv, info = oracle.dgetrf(a)
+    x, _ = oracle.dgetrs(lu, ipiv, b)
+    assert info == int(g["info"]) == 0
+    assert np.array_equal(ipiv, g["ipiv"])
+    assert np.array_equal(x[:, 0].view(np.uint64), g["x"].view(np.uint64))
