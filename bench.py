@@ -37,6 +37,10 @@ def parse():
     p.add_argument("--seed", type=int, default=16384)
     p.add_argument("--cpu-n", type=int, default=2048, help="CPU baseline sample size")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--mp-n", type=int, default=32768, help="order of the mixed-precision comparison")
+    p.add_argument("--no-mp", action="store_true", help="skip the mixed-precision comparison")
+    p.add_argument("--calib-copy-mib", type=int, default=0,
+                   help="run one plain device copy of this many MiB first (calibrates HBM byte counters)")
     return p.parse_args()
 
 
@@ -64,6 +68,71 @@ def cpu_baseline(n_sample, seed):
         "kind": kind,
         "sample": f"DGETRF n={n_sample} N(0,1), reference BLAS, best of 2, {best:.2f} s",
     }
+
+
+def mp_compare(n, seed):
+    """Time-to-FP64-backward-error at order n (A, b ~ N(0,1), nrhs = 1): the plain
+    FP64 DGESV path against the mixed-precision solvers (DSGESV: FP32 LU + FP64
+    refinement; DGESV_IR with FP16 / BF16 trailing updates after DGEEQUB, FP64
+    refinement and GMRES-IR).  Each variant reports its time, ITER, its normwise
+    backward error max|b - A x| / (||A||_inf max|x|) computed in FP64 on the GPU,
+    and its forward error against the FP64 path's solution."""
+    import numpy as np
+    import torch
+    import lapack_amd as la
+    dev = torch.device("cuda:0")
+    print(f"[bench] mixed precision: generating A {n}x{n} N(0,1) seed {seed}", file=sys.stderr, flush=True)
+    a = la.matrix(3, seed, n, n)
+    b = la.matrix(3, seed + 1, n, 1)
+    d_a0 = torch.from_numpy(a.T).to(dev)  # row-major view of the column-major bytes (A^T as a tensor)
+    del a
+    d_b = torch.from_numpy(b.T.copy()).to(dev)
+    rowsum = d_a0.abs().sum(dim=0)  # tensor is A^T: column sums of A^T = row sums of A
+    anorm = rowsum.max().item()
+
+    def berr(x):
+        r = d_b - x @ d_a0  # (1 x n): b^T - x^T A^T
+        return (r.abs().max() / (anorm * x.abs().max())).item()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, (time.perf_counter() - t) * 1e3
+
+    res = {}
+    d_a = torch.empty_like(d_a0)
+    ipiv = torch.zeros(n, dtype=torch.int32, device=dev)
+    # plain FP64 DGETRF + DGETRS (best of 2)
+    best = None
+    for _ in range(2):
+        d_a.copy_(d_a0)
+        x64 = d_b.clone()
+        info, ms = timed(lambda: la.dgesv(n, 1, d_a, n, ipiv, x64, n))
+        assert info == 0, info
+        best = ms if best is None else min(best, ms)
+    res["fp64_dgesv"] = {"ms": round(best, 2), "berr": berr(x64)}
+    xn = x64.abs().max().item()
+    variants = [("fp32_dsgesv", None), ("fp32_ir", "S"), ("fp16_ir", "H"), ("bf16_ir", "B")]
+    for name, prec in variants:
+        best, out = None, None
+        for _ in range(2):
+            d_a.copy_(d_a0)
+            x = torch.zeros_like(d_b)
+            if prec is None:
+                (info, it), ms = timed(lambda: la.dsgesv(n, 1, d_a, n, ipiv, d_b, n, x, n))
+                git = 0
+            else:
+                (info, it, git), ms = timed(lambda: la.dgesv_ir(n, 1, d_a, n, ipiv, d_b, n, x, n, prec))
+            assert info == 0, (name, info)
+            if best is None or ms < best:
+                best, out = ms, (it, git, x.clone())
+        it, git, x = out
+        res[name] = {"ms": round(best, 2), "ratio_to_fp64": round(best / res["fp64_dgesv"]["ms"], 4), "iter": it,
+                     "gmres_iter": git, "berr": berr(x), "fwd_err_vs_fp64": ((x - x64).abs().max().item() / xn)}
+        print(f"[bench] {name}: {res[name]}", file=sys.stderr, flush=True)
+    return {"n": n, "nrhs": 1, "data": f"A, b ~ N(0,1) seeds {seed},{seed + 1}", **res}
 
 
 def main():
@@ -103,6 +172,13 @@ def main():
         rc = la.dgetrs_async("N", n, nrhs, d_as[i], n, d_ipiv[i], d_bs[i], n, stream)
         assert rc == 0, rc
 
+    if args.calib_copy_mib > 0:
+        src = torch.empty(args.calib_copy_mib << 17, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for i in range(W):
         step(K + i)
     torch.cuda.synchronize()
@@ -170,6 +246,10 @@ def main():
         "phase_ms_per_step": {p: round(ph[p][0] / K, 3) for p in ph},
         "dget02_ratio": round(ratio, 4),
     }
+    if not args.no_mp:
+        del d_as, d_bs, d_a0
+        torch.cuda.empty_cache()
+        out["mp_ir"] = mp_compare(args.mp_n, 32768)
     if not args.no_cpu:
         print(f"[bench] CPU baseline sample n={args.cpu_n}", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.seed)

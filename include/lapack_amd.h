@@ -74,6 +74,26 @@ void dsgesv_(const int* n, const int* nrhs, double* a, const int* lda, int* ipiv
              const int* ldb, double* x, const int* ldx, double* work, float* swork, int* iter,
              int* info);
 
+/* DGERFS: iterative refinement of a solution with forward/backward error
+ * bounds, from the factors of DGETRF.  Replaces SRC/dgerfs.f:185.  WORK and
+ * IWORK are accepted for ABI compatibility; FERR/BERR may be host or device. */
+void dgerfs_(const char* trans, const int* n, const int* nrhs, const double* a, const int* lda, const double* af,
+             const int* ldaf, const int* ipiv, const double* b, const int* ldb, double* x, const int* ldx,
+             double* ferr, double* berr, double* work, int* iwork, int* info, size_t trans_len);
+
+/* DGEEQU / DGEEQUB: row and column scale factors (DGEEQUB: powers of the
+ * radix, so scaling is exact).  Replace SRC/dgeequ.f:137 and
+ * SRC/dgeequb.f:144 (LAPACKE/include/lapack.h, dgeequ_/dgeequb_).  R, C may
+ * be host or device arrays; the maxima are taken on the GPU. */
+void dgeequ_(const int* m, const int* n, const double* a, const int* lda, double* r, double* c, double* rowcnd,
+             double* colcnd, double* amax, int* info);
+void dgeequb_(const int* m, const int* n, const double* a, const int* lda, double* r, double* c, double* rowcnd,
+              double* colcnd, double* amax, int* info);
+/* DLAQGE: apply the scaling when ROWCND/COLCND/AMAX call for it; EQUED out.
+ * Replaces SRC/dlaqge.f:140. */
+void dlaqge_(const int* m, const int* n, double* a, const int* lda, const double* r, const double* c,
+             const double* rowcnd, const double* colcnd, const double* amax, char* equed, size_t equed_len);
+
 /* DGEMM: C <- alpha*op(A)*op(B) + beta*C on the FP64 matrix cores.
  * Replaces BLAS/SRC/dgemm.f:213. */
 void dgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k,
@@ -99,6 +119,31 @@ int lapack_amd_dgemm(char transa, char transb, int m, int n, int k, double alpha
 /* DSGESV semantics (SRC/dsgesv.f:259-427); *iter receives ITER. */
 int lapack_amd_dsgesv(int n, int nrhs, double* a, int lda, int* ipiv, const double* b, int ldb, double* x,
                       int ldx, int* iter);
+
+/* Mixed-precision solve to FP64 accuracy.  No single reference routine: it
+ * generalizes DSGESV (SRC/dsgesv.f:259-427; same arguments, same per-column
+ * stopping test max|r| <= max|x|*||A||_inf*eps*sqrt(n), same FP64 LU fallback
+ * and ITER codes -2/-3/-31) to a factorization in PREC:
+ *   'S' FP32 LU;  'H' FP32-stored LU whose trailing updates take FP16
+ *   operands, 'B' BF16 operands (FP32 accumulation on the matrix cores),
+ *   both after DGEEQUB power-of-two equilibration.
+ * Refinement in FP64: classic iterations (corrections solved with the
+ * low-precision factors in FP32) while the backward error drops tenfold per
+ * step, then restarted GMRES on the correction equation preconditioned by
+ * the low-precision LU applied in FP64 (GMRES-IR).  *iter = refinement steps + GMRES cycles, *gmres_iter =
+ * GMRES inner iterations, berr[j] (optional) = max|r_j|/(||A||_inf max|x_j|),
+ * NaN when the FP64 fallback ran (A then holds its factors, as in DSGESV).
+ * Returns INFO (PREC is parameter 10). */
+int lapack_amd_dgesv_ir(int n, int nrhs, double* a, int lda, int* ipiv, const double* b, int ldb, double* x,
+                        int ldx, char prec, int* iter, int* gmres_iter, double* berr);
+int lapack_amd_dgerfs(char trans, int n, int nrhs, const double* a, int lda, const double* af, int ldaf,
+                      const int* ipiv, const double* b, int ldb, double* x, int ldx, double* ferr, double* berr);
+int lapack_amd_dgeequ(int m, int n, const double* a, int lda, double* r, double* c, double* rowcnd, double* colcnd,
+                      double* amax);
+int lapack_amd_dgeequb(int m, int n, const double* a, int lda, double* r, double* c, double* rowcnd,
+                       double* colcnd, double* amax);
+char lapack_amd_dlaqge(int m, int n, double* a, int lda, const double* r, const double* c, double rowcnd,
+                       double colcnd, double amax);
 
 /* ========================= 3. device entry points ========================= */
 

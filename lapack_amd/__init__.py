@@ -60,6 +60,11 @@ def _declare(L):
         "lapack_amd_dgesv": (i, [i, i, vp, i, vp, vp, i]),
         "lapack_amd_dlaswp": (i, [i, vp, i, i, i, vp, i]),
         "lapack_amd_dsgesv": (i, [i, i, vp, i, vp, vp, i, vp, i, _c_int_p]),
+        "lapack_amd_dgesv_ir": (i, [i, i, vp, i, vp, vp, i, vp, i, ch, _c_int_p, _c_int_p, vp]),
+        "lapack_amd_dgerfs": (i, [ch, i, i, vp, i, vp, i, vp, vp, i, vp, i, vp, vp]),
+        "lapack_amd_dgeequ": (i, [i, i, vp, i, vp, vp, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
+        "lapack_amd_dgeequb": (i, [i, i, vp, i, vp, vp, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
+        "lapack_amd_dlaqge": (ch, [i, i, vp, i, vp, vp, d, d, d]),
         "lapack_amd_dgemm": (i, [ch, ch, i, i, i, d, vp, i, vp, i, d, vp, i]),
         "lapack_amd_dgetrf_async": (i, [i, i, vp, i, vp, vp, vp]),
         "lapack_amd_sgetrf_async": (i, [i, i, vp, i, vp, vp, vp]),
@@ -153,6 +158,35 @@ def dsgesv(n, nrhs, a, lda, ipiv, b, ldb, x, ldx):
     it = ctypes.c_int(0)
     info = lib().lapack_amd_dsgesv(n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb, _ptr(x), ldx, ctypes.byref(it))
     return info, it.value
+
+
+def dgesv_ir(n, nrhs, a, lda, ipiv, b, ldb, x, ldx, prec="S", berr=None):
+    """Mixed-precision solve ('S' FP32, 'H' FP16, 'B' BF16 factorization) with FP64
+    refinement and GMRES-IR; returns (INFO, ITER, GMRES_ITER)."""
+    it, git = ctypes.c_int(0), ctypes.c_int(0)
+    info = lib().lapack_amd_dgesv_ir(n, nrhs, _ptr(a), lda, _ptr(ipiv), _ptr(b), ldb, _ptr(x), ldx, _c(prec),
+                                     ctypes.byref(it), ctypes.byref(git), _ptr(berr))
+    return info, it.value, git.value
+
+
+def dgerfs(trans, n, nrhs, a, lda, af, ldaf, ipiv, b, ldb, x, ldx, ferr, berr):
+    """DGERFS (SRC/dgerfs.f:185); X refined in place, FERR/BERR filled; returns INFO."""
+    return lib().lapack_amd_dgerfs(_c(trans), n, nrhs, _ptr(a), lda, _ptr(af), ldaf, _ptr(ipiv), _ptr(b), ldb,
+                                   _ptr(x), ldx, _ptr(ferr), _ptr(berr))
+
+
+def dgeequ(m, n, a, lda, r, c, radix=False):
+    """DGEEQU (radix=False) / DGEEQUB (radix=True); returns (INFO, ROWCND, COLCND, AMAX)."""
+    rc_, cc_, am = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+    fn = lib().lapack_amd_dgeequb if radix else lib().lapack_amd_dgeequ
+    info = fn(m, n, _ptr(a), lda, _ptr(r), _ptr(c), ctypes.byref(rc_), ctypes.byref(cc_), ctypes.byref(am))
+    return info, rc_.value, cc_.value, am.value
+
+
+def dlaqge(m, n, a, lda, r, c, rowcnd, colcnd, amax):
+    """DLAQGE; returns EQUED as a str."""
+    return lib().lapack_amd_dlaqge(m, n, _ptr(a), lda, _ptr(r), _ptr(c), float(rowcnd), float(colcnd),
+                                   float(amax)).decode()
 
 
 def dlaswp(n, a, lda, k1, k2, ipiv, incx):

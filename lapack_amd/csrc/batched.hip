@@ -18,12 +18,14 @@
 //      strict '>', a NaN at the head of the column wins, others never), the
 //      interchange, the scaling by the reciprocal or by division below sfmin
 //      (SRC/dgetrf2.f:196-210) and the rank-1 update of the panel;
-//   2. the panel is written back and its interchanges are applied to the
-//      columns left of it (SRC/dgetrf.f:193);
-//   3. the columns right of it, BC at a time: interchanges (dgetrf.f:199-200),
-//      the unit lower solve U12 = L11^-1 A12 (dgetrf.f:204-207, the loop order
-//      of BLAS/SRC/dtrsm.f:289-300 with its zero test), staged in LDS, then
-//      the update A22 -= A21*U12 (dgetrf.f:212-216) in 4x4 register tiles.
+//   2. the panel is written back and its interchanges are applied to all
+//      other columns (SRC/dgetrf.f:193,199-200): wave 0 first composes the
+//      sequence into (position, source) pairs, then each column moves its
+//      rows with independent loads followed by stores;
+//   3. the columns right of it, BC at a time: the unit lower solve U12 =
+//      L11^-1 A12 (dgetrf.f:204-207, the loop order of BLAS/SRC/dtrsm.f:289-300
+//      with its zero test), staged in LDS, then the update A22 -= A21*U12
+//      (dgetrf.f:212-216) in 4x4 register tiles.
 // DGETRS('N') per matrix: the interchanges on B, then blocked forward and
 // backward substitution -- one wavefront solves each 64-row diagonal block
 // (dtrsm.f:277-300 order, with the reference's zero tests), the workgroup
@@ -79,6 +81,9 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
   __shared__ u64 s_key[NW];
   __shared__ int s_row[NW];
   __shared__ int s_piv[NB];
+  __shared__ T s_pv;  // the pivot value of the current column
+  __shared__ int s_ppos[2 * NB], s_psrc[2 * NB];  // the panel's interchanges as (position, source) pairs
+  __shared__ int s_np;
   __shared__ int s_item, s_info;
   const int tid = threadIdx.x;
   const T sfmin = BSfmin<T>::v;
@@ -118,12 +123,16 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
             if (s_key[q] > bk || (s_key[q] == bk && s_row[q] < br)) { bk = s_key[q]; br = s_row[q]; }
           if (br >= mr) br = j;  // cannot happen (row j is always a candidate); keeps indexes in range
           s_piv[j] = br;
+          s_pv = P[j * ldp + br];
           ipiv[k0 + j] = k0 + br + 1;
-          if (P[j * ldp + br] == T(0) && s_info == 0) s_info = k0 + j + 1;
+          if (s_pv == T(0) && s_info == 0) s_info = k0 + j + 1;
         }
         __syncthreads();
+        // The pivot value comes from LDS written before the barrier: the
+        // interchange below rewrites P(p, j) while other waves may not yet
+        // have read it.
         const int p = s_piv[j];
-        const T pv = P[j * ldp + p];
+        const T pv = s_pv;
         if (pv != T(0)) {
           if (p != j)
             for (int c = tid; c < w; c += NT) {
@@ -146,16 +155,58 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
         }
         __syncthreads();
       }
-      // ---- panel back to HBM; interchanges on the columns to its left
+      // ---- the panel's interchange sequence as (position, source) pairs:
+      // wave 0 replays it over the positions it touches (at most 2w, one
+      // per lane), so every column can then move its rows with independent
+      // loads followed by stores instead of w dependent swaps.
+      if (tid < 64) {
+        const int lane = tid;
+        int mypos = -1, mysrc = -1, cnt = 0;
+        for (int j = 0; j < w; ++j) {
+          const int a = k0 + j, bpos = k0 + s_piv[j];
+          if (a == bpos) continue;
+          const u64 ma = __ballot(mypos == a);
+          int ia;
+          if (ma == 0ull) {
+            if (lane == cnt) { mypos = a; mysrc = a; }
+            ia = cnt++;
+          } else {
+            ia = (int)__builtin_ctzll(ma);
+          }
+          const u64 mb = __ballot(mypos == bpos);
+          int ib;
+          if (mb == 0ull) {
+            if (lane == cnt) { mypos = bpos; mysrc = bpos; }
+            ib = cnt++;
+          } else {
+            ib = (int)__builtin_ctzll(mb);
+          }
+          const int sa = __shfl(mysrc, ia), sb = __shfl(mysrc, ib);
+          if (lane == ia) mysrc = sb;
+          if (lane == ib) mysrc = sa;
+        }
+        if (lane < cnt) { s_ppos[lane] = mypos; s_psrc[lane] = mysrc; }
+        if (lane == 0) s_np = cnt;
+      }
+      __syncthreads();
+      const int np = s_np;
+      // ---- panel back to HBM; interchanges on every other column
       for (int r = tid; r < mr; r += NT)
         for (int c = 0; c < w; ++c) A[(k0 + r) + (long)(k0 + c) * lda] = P[c * ldp + r];
-      for (int c = tid; c < k0; c += NT) {
-        T* col = A + (long)c * lda;
-        for (int j = 0; j < w; ++j) {
-          const int p = k0 + s_piv[j];
-          if (p != k0 + j) { const T t = col[k0 + j]; col[k0 + j] = col[p]; col[p] = t; }
+      if (np > 0) {
+        for (int cc2 = tid; cc2 < n - w; cc2 += NT) {
+          const int c = cc2 < k0 ? cc2 : cc2 + w;
+          T* col = A + (long)c * lda;
+          T tmp[2 * NB];
+#pragma unroll
+          for (int q = 0; q < 2 * NB; ++q)
+            if (q < np) tmp[q] = col[s_psrc[q]];
+#pragma unroll
+          for (int q = 0; q < 2 * NB; ++q)
+            if (q < np) col[s_ppos[q]] = tmp[q];
         }
       }
+      barrier_vm();  // the solves below read rows other threads moved
       // ---- columns to the right, BC at a time
       const int c1 = k0 + w;
       for (int cb = c1; cb < n; cb += BC) {
@@ -163,10 +214,6 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
           const int c = cb + tid;
           if (c < n) {
             T* col = A + (long)c * lda;
-            for (int j = 0; j < w; ++j) {
-              const int p = k0 + s_piv[j];
-              if (p != k0 + j) { const T t = col[k0 + j]; col[k0 + j] = col[p]; col[p] = t; }
-            }
             T x[NB];
 #pragma unroll
             for (int j = 0; j < NB; ++j) x[j] = (j < w) ? col[k0 + j] : T(0);
@@ -186,7 +233,7 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
             for (int j = 0; j < NB; ++j) U[j * BC + tid] = T(0);
           }
         }
-        barrier_vm();  // the update reads rows this phase's interchanges wrote
+        __syncthreads();  // U complete
         const int tx = tid % TX, ty = tid / TX;
         const int cc = ty * TN;  // tile column offset within the block
         for (int rb = c1; rb < n; rb += BR) {
@@ -410,7 +457,7 @@ void launch_fill_uniform_s(float* a, size_t count, unsigned long long seed, hipS
 // Configurations the autotuner chooses from: (NB, threads).
 int batched_getrf_configs() { return 6; }
 void batched_getrf_config(int idx, int* nb, int* nt) {
-  static const int tab[6][2] = {{8, 256}, {16, 256}, {16, 512}, {32, 256}, {32, 512}, {8, 128}};
+  static const int tab[6][2] = {{8, 256}, {16, 256}, {16, 512}, {32, 256}, {16, 128}, {8, 128}};
   *nb = tab[idx][0];
   *nt = tab[idx][1];
 }
@@ -428,7 +475,7 @@ int batched_getrf_grid(int cfg, int batch, int nmax) {
     case 1: return grid_cfg<T, 16, 256>(batch, nmax);
     case 2: return grid_cfg<T, 16, 512>(batch, nmax);
     case 3: return grid_cfg<T, 32, 256>(batch, nmax);
-    case 4: return grid_cfg<T, 32, 512>(batch, nmax);
+    case 4: return grid_cfg<T, 16, 128>(batch, nmax);
     case 5: return grid_cfg<T, 8, 128>(batch, nmax);
   }
   return LA_ERR_LIMIT;
@@ -444,7 +491,7 @@ int launch_getrf_batched(int cfg, int grid, int batch, int nmax, const int* nv, 
     case 1: launch_getrf_cfg<T, 16, 256>(grid, batch, nmax, nv, av, ldav, ipv, infov, order, counter, s); return 0;
     case 2: launch_getrf_cfg<T, 16, 512>(grid, batch, nmax, nv, av, ldav, ipv, infov, order, counter, s); return 0;
     case 3: launch_getrf_cfg<T, 32, 256>(grid, batch, nmax, nv, av, ldav, ipv, infov, order, counter, s); return 0;
-    case 4: launch_getrf_cfg<T, 32, 512>(grid, batch, nmax, nv, av, ldav, ipv, infov, order, counter, s); return 0;
+    case 4: launch_getrf_cfg<T, 16, 128>(grid, batch, nmax, nv, av, ldav, ipv, infov, order, counter, s); return 0;
     case 5: launch_getrf_cfg<T, 8, 128>(grid, batch, nmax, nv, av, ldav, ipv, infov, order, counter, s); return 0;
   }
   return LA_ERR_LIMIT;

@@ -39,6 +39,15 @@ void phase_read(double* ms, long* calls, double* flops);
 void phase_reset();
 int dsgesv_device(int n, int nrhs, double* A, long lda, int* ipiv, const double* B, long ldb, double* X, long ldx,
                   int* iter, hipStream_t s);
+int dgesv_ir_device(int n, int nrhs, double* A, long lda, int* ipiv, const double* B, long ldb, double* X, long ldx,
+                    int kind, int* iter, int* gmres_iter, double* berr, hipStream_t s);
+int geequ_device(int m, int n, const double* A, long lda, double* r, double* c, double* rowcnd, double* colcnd,
+                 double* amax, int radix, double* d_r, double* d_c, hipStream_t s);
+char laqge_device(int m, int n, double* A, long lda, const double* d_r, const double* d_c, double rowcnd,
+                  double colcnd, double amax, hipStream_t s);
+int gerfs_device(char trans, int n, int nrhs, const double* A, long lda, const double* AF, long ldaf,
+                 const int* ipiv, const double* B, long ldb, double* X, long ldx, double* ferr, double* berr,
+                 hipStream_t s);
 }  // namespace la
 
 using namespace la;
@@ -265,6 +274,160 @@ int dsgesv_sync(int n, int nrhs, double* a, int lda, int* ipiv, const double* b,
   return info;
 }
 
+// Mixed-precision solve with refinement / GMRES-IR (no reference routine;
+// arguments checked like DSGESV, SRC/dsgesv.f:263-280, PREC as parameter 10).
+int dgesv_ir_sync(int n, int nrhs, double* a, int lda, int* ipiv, const double* b, int ldb, double* x, int ldx,
+                  char prec, int* iter, int* gmres_iter, double* berr) {
+  *iter = 0;
+  *gmres_iter = 0;
+  if (n < 0) return -1;
+  if (nrhs < 0) return -2;
+  if (lda < std::max(1, n)) return -4;
+  if (ldb < std::max(1, n)) return -7;
+  if (ldx < std::max(1, n)) return -9;
+  int kind;
+  if (lsame(prec, 'S')) kind = 0;
+  else if (lsame(prec, 'H')) kind = 1;
+  else if (lsame(prec, 'B')) kind = 2;
+  else return -10;
+  if (n == 0) return 0;
+  Call call;
+  int rc = call.open();
+  if (rc) return rc;
+  Arr<double> A = arr<double>(a, lda, n, n);
+  Arr<int> P = arr<int>(ipiv, n, n, 1);
+  Arr<double> B = arr<double>(const_cast<double*>(b), ldb, n, nrhs);
+  Arr<double> X = arr<double>(x, ldx, n, nrhs);
+  char* io = nullptr;
+  if ((rc = io_buffer(call.c, A.bytes() + P.bytes() + B.bytes() + X.bytes() + 256, &io))) return rc;
+  char* cur = io;
+  place(A, cur);
+  place(P, cur);
+  place(B, cur);
+  place(X, cur);
+  if ((rc = to_dev(A, call.s)) || (rc = to_dev(B, call.s))) return rc;
+  const int info = dgesv_ir_device(n, nrhs, A.dev, A.ld, P.dev, B.dev, B.ld, X.dev, X.ld, kind, iter, gmres_iter,
+                                   berr, call.s);
+  if (info < 0) return info;
+  if ((rc = to_host(A, call.s)) || (rc = to_host(P, call.s)) || (rc = to_host(X, call.s))) return rc;
+  CK(hipStreamSynchronize(call.s));
+  return info;
+}
+
+// DGERFS (argument checks SRC/dgerfs.f:236-256).
+int gerfs_sync(char trans, int n, int nrhs, const double* a, int lda, const double* af, int ldaf, const int* ipiv,
+               const double* b, int ldb, double* x, int ldx, double* ferr, double* berr) {
+  if (!lsame(trans, 'N') && !is_trans(trans)) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (lda < std::max(1, n)) return -5;
+  if (ldaf < std::max(1, n)) return -7;
+  if (ldb < std::max(1, n)) return -10;
+  if (ldx < std::max(1, n)) return -12;
+  std::vector<double> hf(std::max(nrhs, 1)), hb(std::max(nrhs, 1));
+  const bool fdev = is_device_ptr(ferr), bdev = is_device_ptr(berr);
+  if (n == 0 || nrhs == 0) {
+    for (int j = 0; j < nrhs; ++j) { hf[j] = 0.0; hb[j] = 0.0; }
+  } else {
+    Call call;
+    int rc = call.open();
+    if (rc) return rc;
+    Arr<double> A = arr<double>(const_cast<double*>(a), lda, n, n);
+    Arr<double> F = arr<double>(const_cast<double*>(af), ldaf, n, n);
+    Arr<int> P = arr<int>(const_cast<int*>(ipiv), n, n, 1);
+    Arr<double> B = arr<double>(const_cast<double*>(b), ldb, n, nrhs);
+    Arr<double> X = arr<double>(x, ldx, n, nrhs);
+    char* io = nullptr;
+    if ((rc = io_buffer(call.c, A.bytes() + F.bytes() + P.bytes() + B.bytes() + X.bytes() + 256, &io))) return rc;
+    char* cur = io;
+    place(A, cur);
+    place(F, cur);
+    place(P, cur);
+    place(B, cur);
+    place(X, cur);
+    if ((rc = to_dev(A, call.s)) || (rc = to_dev(F, call.s)) || (rc = to_dev(P, call.s)) ||
+        (rc = to_dev(B, call.s)) || (rc = to_dev(X, call.s)))
+      return rc;
+    const char tr = lsame(trans, 'N') ? 'N' : 'T';
+    if ((rc = gerfs_device(tr, n, nrhs, A.dev, A.ld, F.dev, F.ld, P.dev, B.dev, B.ld, X.dev, X.ld, hf.data(),
+                           hb.data(), call.s)))
+      return rc;
+    if ((rc = to_host(X, call.s))) return rc;
+    CK(hipStreamSynchronize(call.s));
+  }
+  if (nrhs > 0) {
+    if (fdev) CK(hipMemcpy(ferr, hf.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice));
+    else std::memcpy(ferr, hf.data(), sizeof(double) * nrhs);
+    if (bdev) CK(hipMemcpy(berr, hb.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice));
+    else std::memcpy(berr, hb.data(), sizeof(double) * nrhs);
+  }
+  return 0;
+}
+
+// DGEEQU / DGEEQUB (SRC/dgeequ.f:163-173, SRC/dgeequb.f:170-180 argument checks).
+int geequ_sync(int m, int n, const double* a, int lda, double* r, double* c, double* rowcnd, double* colcnd,
+               double* amax, int radix) {
+  if (m < 0) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max(1, m)) return -4;
+  if (m == 0 || n == 0) {
+    *rowcnd = 1.0;
+    *colcnd = 1.0;
+    *amax = 0.0;
+    return 0;
+  }
+  Call call;
+  int rc = call.open();
+  if (rc) return rc;
+  Arr<double> A = arr<double>(const_cast<double*>(a), lda, m, n);
+  char* io = nullptr;
+  if ((rc = io_buffer(call.c, A.bytes() + al256(sizeof(double) * (size_t)(m + n)) + 256, &io))) return rc;
+  char* cur = io;
+  double* d_r = reinterpret_cast<double*>(cur);
+  double* d_c = d_r + m;
+  cur += al256(sizeof(double) * (size_t)(m + n));
+  place(A, cur);
+  if ((rc = to_dev(A, call.s))) return rc;
+  const bool rdev = is_device_ptr(r), cdev = is_device_ptr(c);
+  std::vector<double> hr(m), hc(n);
+  const int info = geequ_device(m, n, A.dev, A.ld, hr.data(), hc.data(), rowcnd, colcnd, amax, radix, d_r, d_c,
+                                call.s);
+  if (info < 0) return info;
+  // R and C are left as the reference leaves them: on a zero row (INFO <= M)
+  // R holds the row maxima and C is untouched; on a zero column R is final
+  // and C holds the column maxima.
+  if (rdev) CK(hipMemcpy(r, hr.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+  else std::memcpy(r, hr.data(), sizeof(double) * m);
+  if (info == 0 || info > m) {
+    if (cdev) CK(hipMemcpy(c, hc.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    else std::memcpy(c, hc.data(), sizeof(double) * n);
+  }
+  return info;
+}
+
+char laqge_sync(int m, int n, double* a, int lda, const double* r, const double* c, double rowcnd, double colcnd,
+                double amax, int* err) {
+  *err = 0;
+  if (m <= 0 || n <= 0) return 'N';
+  Call call;
+  int rc = call.open();
+  if (rc) { *err = rc; return 'N'; }
+  Arr<double> A = arr<double>(a, lda, m, n);
+  Arr<double> R = arr<double>(const_cast<double*>(r), m, m, 1);
+  Arr<double> C = arr<double>(const_cast<double*>(c), n, n, 1);
+  char* io = nullptr;
+  if ((rc = io_buffer(call.c, A.bytes() + R.bytes() + C.bytes() + 256, &io))) { *err = rc; return 'N'; }
+  char* cur = io;
+  place(A, cur);
+  place(R, cur);
+  place(C, cur);
+  if ((rc = to_dev(A, call.s)) || (rc = to_dev(R, call.s)) || (rc = to_dev(C, call.s))) { *err = rc; return 'N'; }
+  const char equed = laqge_device(m, n, A.dev, A.ld, R.dev, C.dev, rowcnd, colcnd, amax, call.s);
+  if (equed != 'N' && (rc = to_host(A, call.s))) { *err = rc; return equed; }
+  if ((rc = hipc(hipStreamSynchronize(call.s)))) *err = rc;
+  return equed;
+}
+
 template <class T>
 int laswp_sync(int n, T* a, int lda, int k1, int k2, const int* ipiv, int incx) {
   if (n <= 0 || incx == 0) return 0;
@@ -424,6 +587,27 @@ void dsgesv_(const int* n, const int* nrhs, double* a, const int* lda, int* ipiv
   *info = dsgesv_sync(*n, *nrhs, a, *lda, ipiv, b, *ldb, x, *ldx, iter);
   if (*info < 0 && *info > -1000) call_xerbla("DSGESV", *info);
 }
+void dgerfs_(const char* trans, const int* n, const int* nrhs, const double* a, const int* lda, const double* af,
+             const int* ldaf, const int* ipiv, const double* b, const int* ldb, double* x, const int* ldx,
+             double* ferr, double* berr, double*, int*, int* info, size_t) {
+  *info = gerfs_sync(*trans, *n, *nrhs, a, *lda, af, *ldaf, ipiv, b, *ldb, x, *ldx, ferr, berr);
+  if (*info < 0 && *info > -1000) call_xerbla("DGERFS", *info);
+}
+void dgeequ_(const int* m, const int* n, const double* a, const int* lda, double* r, double* c, double* rowcnd,
+             double* colcnd, double* amax, int* info) {
+  *info = geequ_sync(*m, *n, a, *lda, r, c, rowcnd, colcnd, amax, 0);
+  if (*info < 0 && *info > -1000) call_xerbla("DGEEQU", *info);
+}
+void dgeequb_(const int* m, const int* n, const double* a, const int* lda, double* r, double* c, double* rowcnd,
+              double* colcnd, double* amax, int* info) {
+  *info = geequ_sync(*m, *n, a, *lda, r, c, rowcnd, colcnd, amax, 1);
+  if (*info < 0 && *info > -1000) call_xerbla("DGEEQUB", *info);
+}
+void dlaqge_(const int* m, const int* n, double* a, const int* lda, const double* r, const double* c,
+             const double* rowcnd, const double* colcnd, const double* amax, char* equed, size_t) {
+  int err = 0;
+  *equed = laqge_sync(*m, *n, a, *lda, r, c, *rowcnd, *colcnd, *amax, &err);
+}
 void dlaswp_(const int* n, double* a, const int* lda, const int* k1, const int* k2, const int* ipiv,
              const int* incx) {
   (void)laswp_sync<double>(*n, a, *lda, *k1, *k2, ipiv, *incx);
@@ -450,6 +634,27 @@ int lapack_amd_dgesv(int n, int nrhs, double* a, int lda, int* ipiv, double* b, 
 int lapack_amd_dsgesv(int n, int nrhs, double* a, int lda, int* ipiv, const double* b, int ldb, double* x,
                       int ldx, int* iter) {
   return dsgesv_sync(n, nrhs, a, lda, ipiv, b, ldb, x, ldx, iter);
+}
+int lapack_amd_dgesv_ir(int n, int nrhs, double* a, int lda, int* ipiv, const double* b, int ldb, double* x, int ldx,
+                        char prec, int* iter, int* gmres_iter, double* berr) {
+  return dgesv_ir_sync(n, nrhs, a, lda, ipiv, b, ldb, x, ldx, prec, iter, gmres_iter, berr);
+}
+int lapack_amd_dgerfs(char trans, int n, int nrhs, const double* a, int lda, const double* af, int ldaf,
+                      const int* ipiv, const double* b, int ldb, double* x, int ldx, double* ferr, double* berr) {
+  return gerfs_sync(trans, n, nrhs, a, lda, af, ldaf, ipiv, b, ldb, x, ldx, ferr, berr);
+}
+int lapack_amd_dgeequ(int m, int n, const double* a, int lda, double* r, double* c, double* rowcnd, double* colcnd,
+                      double* amax) {
+  return geequ_sync(m, n, a, lda, r, c, rowcnd, colcnd, amax, 0);
+}
+int lapack_amd_dgeequb(int m, int n, const double* a, int lda, double* r, double* c, double* rowcnd, double* colcnd,
+                       double* amax) {
+  return geequ_sync(m, n, a, lda, r, c, rowcnd, colcnd, amax, 1);
+}
+char lapack_amd_dlaqge(int m, int n, double* a, int lda, const double* r, const double* c, double rowcnd,
+                       double colcnd, double amax) {
+  int err = 0;
+  return laqge_sync(m, n, a, lda, r, c, rowcnd, colcnd, amax, &err);
 }
 int lapack_amd_dlaswp(int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx) {
   return laswp_sync<double>(n, a, lda, k1, k2, ipiv, incx);

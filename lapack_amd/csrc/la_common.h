@@ -41,6 +41,10 @@ void launch_dgemm(char ta, char tb, int m, int n, int k, double alpha, const dou
 // skip_xcd0: leave XCD 0 to a concurrent panel kernel.
 void launch_dgemm_update(int m, int n, int k, const double* A, long lda, const double* B, long ldb, double* C,
                          long ldc, bool skip_xcd0, hipStream_t s);
+// FP32 update with operands rounded to FP16 (kind 1) or BF16 (kind 2), FP32
+// accumulation on the half-precision matrix cores (gemm_h.hip).
+void launch_sgemm_update_half(int kind, int m, int n, int k, const float* A, long lda, const float* B, long ldb,
+                              float* C, long ldc, bool skip0, hipStream_t s);
 void launch_sgemm_update(int m, int n, int k, const float* A, long lda, const float* B, long ldb, float* C, long ldc,
                          bool skip_xcd0, hipStream_t s);
 // Same in FP32 (MFMA f32).
@@ -122,6 +126,22 @@ void launch_row_abssum(int m, int n, const double* A, long lda, double* rowsum, 
 void launch_row_absmax(int m, int n, const double* A, long lda, double* r, hipStream_t s);
 void launch_col_absmax_scaled(int m, int n, const double* A, long lda, const double* r, double* c, hipStream_t s);
 void launch_scale_rc(int m, int n, double* A, long lda, const double* r, const double* c, int mode, hipStream_t s);
+void launch_scale_cast_d2s(int m, int n, const double* A, long lda, const double* r, const double* c, float* S,
+                           long lds, int* ovf, hipStream_t s);
+void launch_scale_s2d(int m, int n, const float* S, long lds, const double* c, double* Y, long ldy, int add,
+                      hipStream_t s);
+void launch_vdiv(int n, const double* x, const double* nrm, double* y, hipStream_t s);
+
+// Skinny FP64 products (gemv.hip); ws: gemv_ws_doubles(m, n, r) doubles.
+size_t gemv_ws_doubles(int m, int n, int r);
+void launch_dgemv_n(int m, int n, int r, double alpha, const double* A, long lda, const double* X, long ldx,
+                    double beta, double* Y, long ldy, double* ws, hipStream_t s);
+void launch_dgemv_t(int m, int k, double alpha, const double* V, long ldv, const double* w, double beta, double* h,
+                    double* ws, hipStream_t s);
+void launch_dgemv_cols(int m, int n, double alpha, const double* A, long lda, const double* x, double beta,
+                       double* y, bool abs, hipStream_t s);
+void launch_dgemv_n_abs(int m, int n, double alpha, const double* A, long lda, const double* X, double beta,
+                        double* Y, double* ws, hipStream_t s);
 
 // Largest LDS allocation of one workgroup on the current device (bytes).
 int lds_limit();
@@ -145,6 +165,15 @@ template <class T>
 int launch_getrs_batched(int grid, int batch, int nmax, const int* nv, const int* nrhsv, T* const* av,
                          const int* ldav, int* const* ipv, T* const* bv, const int* ldbv, const int* order,
                          unsigned* counter, hipStream_t s);
+
+// Triangular sweeps of DGETRS in one launch each (getrs.hip).  which: 0 = L
+// (unit lower), 1 = U, 2 = U^T, 3 = L^T; X is n x nrhs (ldx), solved in place.
+// ctl: tri_chain_ctl_words(n) unsigned of scratch (cleared by the launcher);
+// err: set to 1 if a wait timed out.
+size_t tri_chain_ctl_words(int n);
+template <class T>
+void launch_tri_chain(int which, int n, int nrhs, const T* A, long lda, T* X, long ldx, unsigned* ctl, int* err,
+                      hipStream_t s);
 
 // Rank-k outer product C <- alpha*A*B^T + beta*C for k <= rankk_max()
 // (adjoint.hip), the Abar = -Bbar X^T step of the solve's adjoint.

@@ -210,6 +210,11 @@ template <> struct K<double> {
     return launch_getf2_panel_d(T, ldt, j0, w, m, ipiv, info, tagbase, x, tags, err, rk, xcd0, s);
   }
 };
+// Operand precision of the FP32 factorization's trailing updates on the
+// calling thread: 0 = FP32, 1 = FP16, 2 = BF16 (set by getrf_device_lp for
+// the duration of one factorization).
+static thread_local int t_upd_kind = 0;
+
 template <> struct K<float> {
   static void gemm(char ta, char tb, int m, int n, int k, float al, const float* A, long lda, const float* B,
                    long ldb, float be, float* C, long ldc, hipStream_t s) {
@@ -217,7 +222,10 @@ template <> struct K<float> {
   }
   static void update(int m, int n, int k, const float* A, long lda, const float* B, long ldb, float* C, long ldc,
                      bool skip0, hipStream_t s) {
-    launch_sgemm_update(m, n, k, A, lda, B, ldb, C, ldc, skip0, s);
+    if (t_upd_kind == 1 || t_upd_kind == 2)
+      launch_sgemm_update_half(t_upd_kind, m, n, k, A, lda, B, ldb, C, ldc, skip0, s);
+    else
+      launch_sgemm_update(m, n, k, A, lda, B, ldb, C, ldc, skip0, s);
   }
   static void transpose(int r, int c, const float* in, long li, float* out, long lo, hipStream_t s) {
     launch_transpose_s(r, c, in, li, out, lo, s);
@@ -407,6 +415,8 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
   auto Tp = [&](long r, long col) { return Tm + r + col * ldt; };
 
   launch_fill_u32(tags, panel_tag_count(), 0u, s);
+  // the panel's flag-in-data exchange needs a zeroed buffer per factorization
+  launch_fill_u32(reinterpret_cast<unsigned*>(xchg), panel_xchg_bytes(0) / sizeof(unsigned), 0u, s);
   launch_fill_u32(rankctr, nrank, 0u, s);
   launch_fill_i32(err, 1, 0, s);
   launch_fill_i32(info, 1, 0, s);
@@ -537,7 +547,20 @@ size_t getrs_ws_bytes(int n, int nrhs, int nbs) {
   cv.take<T>((size_t)nbs * nrhs);
   cv.take<T>((size_t)nbs * (size_t)((n + nbs - 1) / nbs) * nbs);
   cv.take<T>((size_t)nbs * (size_t)((n + nbs - 1) / nbs) * nbs);
+  cv.take<unsigned>(tri_chain_ctl_words(n));
+  cv.take<int>(1);
   return cv.off;
+}
+
+// The blocked-GEMM form of the sweeps (diagonal-block inverses + MFMA
+// products) is kept for comparison: LAPACK_AMD_GETRS=gemm selects it.
+static bool getrs_use_gemm() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("LAPACK_AMD_GETRS");
+    v = (e && std::strcmp(e, "gemm") == 0) ? 1 : 0;
+  }
+  return v != 0;
 }
 
 // Solve op(A) X = B with the LU of getrf (SRC/dgetrs.f:181-218).  Device
@@ -562,6 +585,8 @@ int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* i
   T* Y = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nbs * nrhs));
   T* WL = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nbs * nblk * nbs));
   T* WU = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nbs * nblk * nbs));
+  unsigned* ctl = reinterpret_cast<unsigned*>(ws + cv.take<unsigned>(tri_chain_ctl_words(n)));
+  int* cerr = reinterpret_cast<int*>(ws + cv.take<int>(1));
   const long ldx = n, ldy = nbs, ldw = nbs;
   auto Ap = [&](long r, long col) { return A + r + col * lda; };
 
@@ -572,6 +597,27 @@ int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* i
   pt.mark(LA_PHASE_GETRS);
   fl.push_back(2.0 * (double)n * n * nrhs);
 
+  if (!getrs_use_gemm()) {
+    launch_fill_i32(cerr, 1, 0, s);
+    if (notran) {
+      launch_swaps_to_perm(n, 0, n, ipiv, 1, perm, s);
+      K<T>::gather(n, nrhs, perm, B, ldb, X, ldx, s);                // X = P*B (dgetrs.f:187)
+      launch_tri_chain<T>(0, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // L Y = X (dgetrs.f:191)
+      launch_tri_chain<T>(1, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // U X = Y (dgetrs.f:197)
+      K<T>::copy(n, nrhs, X, ldx, B, ldb, s);
+    } else {
+      K<T>::copy(n, nrhs, B, ldb, X, ldx, s);
+      launch_tri_chain<T>(2, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // U^T Y = B (dgetrs.f:205)
+      launch_tri_chain<T>(3, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // L^T X = Y (dgetrs.f:211)
+      launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s);               // dgetrs.f:217
+      K<T>::gather(n, nrhs, perm, X, ldx, B, ldb, s);
+    }
+    pt.mark(-1);
+    fl.push_back(0);
+    rc = ws_release(c, s);
+    phase_collect(pt, fl);
+    return rc;
+  }
   K<T>::trtri_blocks(false, true, n, nbs, A, lda, WL, ldw, s);   // inv(L_dd), unit lower
   K<T>::trtri_blocks(true, false, n, nbs, A, lda, WU, ldw, s);   // inv(U_dd), non-unit upper
   if (notran) {
@@ -644,6 +690,15 @@ int laswp_device(int ncols, T* A, long lda, int nrows, int k1, int k2, const int
   K<T>::gather(nrows, ncols, perm, A, lda, X, nrows, s);
   K<T>::copy(nrows, ncols, X, nrows, A, lda, s);
   return ws_release(c, s);
+}
+
+// FP32-stored LU whose trailing updates take FP16 (kind 1) or BF16 (kind 2)
+// operands with FP32 accumulation; kind 0 is plain SGETRF.
+int getrf_device_lp(int m, int n, float* A, long lda, int* ipiv, int* info, int kind, hipStream_t s) {
+  t_upd_kind = kind;
+  const int rc = getrf_device<float>(m, n, A, lda, ipiv, info, s);
+  t_upd_kind = 0;
+  return rc;
 }
 
 // Explicit instantiations used by the C ABI and the mixed-precision solver.

@@ -122,7 +122,60 @@ __global__ __launch_bounds__(256) void scale_rc_kernel(int m, int n, double* __r
   }
 }
 
+// S(i,j) = REAL(r(i) * c(j) * A(i,j)) (r, c optional, powers of two in the
+// equilibrated solver, so the scaling itself is exact); overflow flag as DLAG2S.
+__global__ __launch_bounds__(256) void scale_cast_d2s_kernel(int m, int n, const double* __restrict__ A, long lda,
+                                                             const double* __restrict__ r,
+                                                             const double* __restrict__ c, float* __restrict__ S,
+                                                             long lds, int* __restrict__ ovf) {
+  const long tot = (long)m * n;
+  bool bad = false;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
+    const long i = e % m, j = e / m;
+    double v = A[i + j * lda];
+    if (r) v = r[i] * v;
+    if (c) v = c[j] * v;
+    if (v < -3.4028234663852886e38 || v > 3.4028234663852886e38) bad = true;
+    S[i + j * lds] = (float)v;
+  }
+  if (bad && ovf) atomicOr(ovf, 1);
+}
+
+// Y(i,j) = c(i) * S(i,j)  (+ Y(i,j) when add): back to FP64, column scaling
+// of the equilibrated system applied to the solution (c optional).
+__global__ __launch_bounds__(256) void scale_s2d_kernel(int m, int n, const float* __restrict__ S, long lds,
+                                                        const double* __restrict__ c, double* __restrict__ Y,
+                                                        long ldy, int add) {
+  const long tot = (long)m * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
+    const long i = e % m, j = e / m;
+    double v = (double)S[i + j * lds];
+    if (c) v = c[i] * v;
+    double* yp = Y + i + j * ldy;
+    *yp = add ? *yp + v : v;
+  }
+}
+
+// y = x / *nrm (GMRES basis vector); nrm is a device scalar.
+__global__ __launch_bounds__(256) void vdiv_kernel(int n, const double* __restrict__ x,
+                                                   const double* __restrict__ nrm, double* __restrict__ y) {
+  const double d = *nrm;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) y[i] = x[i] / d;
+}
+
 }  // namespace
+
+void launch_scale_cast_d2s(int m, int n, const double* A, long lda, const double* r, const double* c, float* S,
+                           long lds, int* ovf, hipStream_t s) {
+  if (m > 0 && n > 0) scale_cast_d2s_kernel<<<grid_for((long)m * n), 256, 0, s>>>(m, n, A, lda, r, c, S, lds, ovf);
+}
+void launch_scale_s2d(int m, int n, const float* S, long lds, const double* c, double* Y, long ldy, int add,
+                      hipStream_t s) {
+  if (m > 0 && n > 0) scale_s2d_kernel<<<grid_for((long)m * n), 256, 0, s>>>(m, n, S, lds, c, Y, ldy, add);
+}
+void launch_vdiv(int n, const double* x, const double* nrm, double* y, hipStream_t s) {
+  if (n > 0) vdiv_kernel<<<grid_for(n), 256, 0, s>>>(n, x, nrm, y);
+}
 
 void launch_cast_d2s(int m, int n, const double* A, long lda, float* S, long lds, int* ovf, hipStream_t s) {
   if (m > 0 && n > 0) cast_d2s_kernel<<<grid_for((long)m * n), 256, 0, s>>>(m, n, A, lda, S, lds, ovf);

@@ -61,6 +61,7 @@ def o():
             "oracle_dget01": (d, [i, i, vp, i, vp, i, vp]),
             "oracle_dget02": (d, [ch, i, i, i, vp, i, vp, i, vp, i]),
             "oracle_dget08": (d, [ch, i, i, i, vp, i, vp, i, vp, i]),
+            "oracle_dgerfs": (i, [ch, i, i, vp, i, vp, i, vp, vp, i, vp, i, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -244,6 +245,21 @@ def dget08(a, x, b, trans="N"):
                              _p(b), max(1, b.shape[0]))
 
 
+def dgerfs(a, lu, ipiv, b, x, trans="N"):
+    """(X refined, FERR, BERR, INFO) of the reference DGERFS on factors lu/ipiv."""
+    a = np.asfortranarray(a, dtype=np.float64)
+    lu = np.asfortranarray(lu, dtype=np.float64)
+    b = np.asfortranarray(b, dtype=np.float64).reshape(a.shape[0], -1, order="F")
+    xx = np.array(x, dtype=np.float64, order="F", copy=True).reshape(a.shape[0], -1, order="F")
+    n, nrhs = b.shape
+    ferr = np.zeros(max(1, nrhs))
+    berr = np.zeros(max(1, nrhs))
+    ip = np.ascontiguousarray(ipiv, dtype=np.int32)
+    info = o().oracle_dgerfs(trans.encode(), n, nrhs, _p(a), max(1, n), _p(lu), max(1, n), _p(ip), _p(b),
+                             max(1, n), _p(xx), max(1, n), _p(ferr), _p(berr))
+    return xx, ferr[:nrhs], berr[:nrhs], info
+
+
 # ---- the flang-built reference (Fortran ABI) ---------------------------------
 
 def _ri(v):
@@ -300,6 +316,25 @@ def ref_dsgesv(a, b):
                 ctypes.c_void_p(_p(bb)), _ri(max(1, n)), ctypes.c_void_p(_p(x)), _ri(max(1, n)),
                 ctypes.c_void_p(_p(work)), ctypes.c_void_p(_p(swork)), ctypes.byref(it), ctypes.byref(info))
     return x, ipiv[:n], it.value, info.value, aa
+
+
+def ref_dgerfs(a, lu, ipiv, b, x, trans="N"):
+    a = np.asfortranarray(a, dtype=np.float64)
+    lu = np.asfortranarray(lu, dtype=np.float64)
+    b = np.asfortranarray(b, dtype=np.float64).reshape(a.shape[0], -1, order="F")
+    xx = np.array(x, dtype=np.float64, order="F", copy=True).reshape(a.shape[0], -1, order="F")
+    n, nrhs = b.shape
+    ferr = np.zeros(max(1, nrhs))
+    berr = np.zeros(max(1, nrhs))
+    work = np.zeros(3 * max(1, n))
+    iwork = np.zeros(max(1, n), dtype=np.int32)
+    ip = np.ascontiguousarray(ipiv, dtype=np.int32)
+    info = ctypes.c_int(0)
+    vp = ctypes.c_void_p
+    r().dgerfs_(ctypes.c_char_p(trans.encode()), _ri(n), _ri(nrhs), vp(_p(a)), _ri(max(1, n)), vp(_p(lu)),
+                _ri(max(1, n)), vp(_p(ip)), vp(_p(b)), _ri(max(1, n)), vp(_p(xx)), _ri(max(1, n)), vp(_p(ferr)),
+                vp(_p(berr)), vp(_p(work)), vp(_p(iwork)), ctypes.byref(info), ctypes.c_size_t(1))
+    return xx, ferr[:nrhs], berr[:nrhs], info.value
 
 
 def ref_dlaswp(a, k1, k2, ipiv, incx=1):

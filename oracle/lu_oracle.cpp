@@ -24,6 +24,8 @@
 //     equality with _ref's DSGESV (ITER, IPIV, X).
 //   geequ / geequb / laqge                                     -- DCHKEQ
 //     closed form (TESTING/LIN/dchkeq.f:107-173) and equality with _ref.
+//   gemv, asum, lacn2, gerfs                                   -- bitwise
+//     equality with _ref's DGERFS (X, FERR, BERR) and DGET07's bounds.
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -338,6 +340,217 @@ void lag2d(int m, int n, const float* sa, int ldsa, double* a, int lda) {
     for (int i = 1; i <= m; ++i) AT(a, i, j, lda) = (double)AT(sa, i, j, ldsa);
 }
 
+// DGEMV y <- alpha*op(A)*x + beta*y, incx = incy = 1 (BLAS/SRC/dgemv.f:254-330):
+// 'N' as column axpys, 'T' as column dot products.
+void gemv(char trans, int m, int n, double alpha, const double* a, int lda, const double* x, double beta,
+          double* y) {
+  const bool nt = (trans == 'N' || trans == 'n');
+  const int leny = nt ? m : n, lenx = nt ? n : m;
+  if (m == 0 || n == 0 || (alpha == 0.0 && beta == 1.0)) return;
+  if (beta != 1.0) {
+    for (int i = 0; i < leny; ++i) y[i] = (beta == 0.0) ? 0.0 : beta * y[i];
+  }
+  if (alpha == 0.0) return;
+  (void)lenx;
+  if (nt) {
+    for (int j = 1; j <= n; ++j) {
+      const double temp = alpha * x[j - 1];
+      for (int i = 1; i <= m; ++i) y[i - 1] = y[i - 1] + temp * AT(a, i, j, lda);
+    }
+  } else {
+    for (int j = 1; j <= n; ++j) {
+      double temp = 0.0;
+      for (int i = 1; i <= m; ++i) temp = temp + AT(a, i, j, lda) * x[i - 1];
+      y[j - 1] = y[j - 1] + alpha * temp;
+    }
+  }
+}
+
+// DASUM, incx = 1: the reference's 6-way unrolled loop adds left to right
+// (BLAS/SRC/dasum.f:104-120), i.e. in index order.
+double asum(int n, const double* x) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s = s + std::fabs(x[i]);
+  return s;
+}
+
+// DLACN2 (SRC/dlacn2.f:167-282): Hager/Higham 1-norm estimator by reverse
+// communication; isave[0..2] is ISAVE(1..3).
+void lacn2(int n, double* v, double* x, int* isgn, double* est, int* kase, int* isave) {
+  const int itmax = 5;
+  if (*kase == 0) {
+    for (int i = 0; i < n; ++i) x[i] = 1.0 / (double)n;
+    *kase = 1;
+    isave[0] = 1;
+    return;
+  }
+  auto main_loop = [&]() {  // label 50
+    for (int i = 0; i < n; ++i) x[i] = 0.0;
+    x[isave[1] - 1] = 1.0;
+    *kase = 1;
+    isave[0] = 3;
+  };
+  auto final_stage = [&]() {  // label 120
+    double altsgn = 1.0;
+    for (int i = 1; i <= n; ++i) {
+      x[i - 1] = altsgn * (1.0 + (double)(i - 1) / (double)(n - 1));
+      altsgn = -altsgn;
+    }
+    *kase = 1;
+    isave[0] = 5;
+  };
+  switch (isave[0]) {
+    case 1: {  // label 20
+      if (n == 1) {
+        v[0] = x[0];
+        *est = std::fabs(v[0]);
+        *kase = 0;
+        return;
+      }
+      *est = asum(n, x);
+      for (int i = 0; i < n; ++i) {
+        x[i] = (x[i] >= 0.0) ? 1.0 : -1.0;
+        isgn[i] = (int)std::lround(x[i]);
+      }
+      *kase = 2;
+      isave[0] = 2;
+      return;
+    }
+    case 2: {  // label 40
+      isave[1] = iamax(n, x, 1);
+      isave[2] = 2;
+      main_loop();
+      return;
+    }
+    case 3: {  // label 70
+      for (int i = 0; i < n; ++i) v[i] = x[i];
+      const double estold = *est;
+      *est = asum(n, v);
+      bool same = true;
+      for (int i = 0; i < n; ++i) {
+        const double xs = (x[i] >= 0.0) ? 1.0 : -1.0;
+        if ((int)std::lround(xs) != isgn[i]) { same = false; break; }
+      }
+      if (same || *est <= estold) { final_stage(); return; }
+      for (int i = 0; i < n; ++i) {
+        x[i] = (x[i] >= 0.0) ? 1.0 : -1.0;
+        isgn[i] = (int)std::lround(x[i]);
+      }
+      *kase = 2;
+      isave[0] = 4;
+      return;
+    }
+    case 4: {  // label 110
+      const int jlast = isave[1];
+      isave[1] = iamax(n, x, 1);
+      if (x[jlast - 1] != std::fabs(x[isave[1] - 1]) && isave[2] < itmax) {
+        isave[2] = isave[2] + 1;
+        main_loop();
+        return;
+      }
+      final_stage();
+      return;
+    }
+    case 5: {  // label 140
+      const double temp = 2.0 * (asum(n, x) / (double)(3 * n));
+      if (temp > *est) {
+        for (int i = 0; i < n; ++i) v[i] = x[i];
+        *est = temp;
+      }
+      *kase = 0;
+      return;
+    }
+  }
+}
+
+// DGERFS (SRC/dgerfs.f:262-432): refinement with ITMAX = 5, componentwise
+// backward error BERR and the DLACN2 bound FERR, column by column.
+int gerfs(char trans, int n, int nrhs, const double* a, int lda, const double* af, int ldaf, const int* ipiv,
+          const double* b, int ldb, double* x, int ldx, double* ferr, double* berr) {
+  const bool notran = (trans == 'N' || trans == 'n');
+  if (!notran && !(trans == 'T' || trans == 't' || trans == 'C' || trans == 'c')) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (lda < std::max(1, n)) return -5;
+  if (ldaf < std::max(1, n)) return -7;
+  if (ldb < std::max(1, n)) return -10;
+  if (ldx < std::max(1, n)) return -12;
+  if (n == 0 || nrhs == 0) {
+    for (int j = 0; j < nrhs; ++j) { ferr[j] = 0.0; berr[j] = 0.0; }
+    return 0;
+  }
+  const char transt = notran ? 'T' : 'N';
+  const int itmax = 5;
+  const int nz = n + 1;
+  const double eps = Mach<double>::eps;
+  const double safmin = Mach<double>::sfmin;
+  const double safe1 = nz * safmin;
+  const double safe2 = safe1 / eps;
+  std::vector<double> work(3 * (size_t)n);
+  std::vector<int> iwork(n);
+  double* w1 = work.data();
+  double* w2 = work.data() + n;
+  double* w3 = work.data() + 2 * (size_t)n;
+  for (int j = 1; j <= nrhs; ++j) {
+    double* xj = &AT(x, 1, j, ldx);
+    const double* bj = &AT(b, 1, j, ldb);
+    int count = 1;
+    double lstres = 3.0;
+    while (true) {
+      for (int i = 0; i < n; ++i) w2[i] = bj[i];
+      gemv(trans, n, n, -1.0, a, lda, xj, 1.0, w2);
+      for (int i = 0; i < n; ++i) w1[i] = std::fabs(bj[i]);
+      if (notran) {
+        for (int k = 1; k <= n; ++k) {
+          const double xk = std::fabs(xj[k - 1]);
+          for (int i = 1; i <= n; ++i) w1[i - 1] = w1[i - 1] + std::fabs(AT(a, i, k, lda)) * xk;
+        }
+      } else {
+        for (int k = 1; k <= n; ++k) {
+          double sacc = 0.0;
+          for (int i = 1; i <= n; ++i) sacc = sacc + std::fabs(AT(a, i, k, lda)) * std::fabs(xj[i - 1]);
+          w1[k - 1] = w1[k - 1] + sacc;
+        }
+      }
+      double sb = 0.0;
+      for (int i = 0; i < n; ++i) {
+        if (w1[i] > safe2) sb = std::max(sb, std::fabs(w2[i]) / w1[i]);
+        else sb = std::max(sb, (std::fabs(w2[i]) + safe1) / (w1[i] + safe1));
+      }
+      berr[j - 1] = sb;
+      if (berr[j - 1] > eps && 2.0 * berr[j - 1] <= lstres && count <= itmax) {
+        getrs(trans, n, 1, af, ldaf, ipiv, w2, n);
+        for (int i = 0; i < n; ++i) xj[i] = xj[i] + 1.0 * w2[i];
+        lstres = berr[j - 1];
+        count = count + 1;
+        continue;
+      }
+      break;
+    }
+    for (int i = 0; i < n; ++i) {
+      if (w1[i] > safe2) w1[i] = std::fabs(w2[i]) + nz * eps * w1[i];
+      else w1[i] = std::fabs(w2[i]) + nz * eps * w1[i] + safe1;
+    }
+    int kase = 0;
+    int isave[3] = {0, 0, 0};
+    while (true) {
+      lacn2(n, w3, w2, iwork.data(), &ferr[j - 1], &kase, isave);
+      if (kase == 0) break;
+      if (kase == 1) {
+        getrs(transt, n, 1, af, ldaf, ipiv, w2, n);
+        for (int i = 0; i < n; ++i) w2[i] = w1[i] * w2[i];
+      } else {
+        for (int i = 0; i < n; ++i) w2[i] = w1[i] * w2[i];
+        getrs(trans, n, 1, af, ldaf, ipiv, w2, n);
+      }
+    }
+    lstres = 0.0;
+    for (int i = 0; i < n; ++i) lstres = std::max(lstres, std::fabs(xj[i]));
+    if (lstres != 0.0) ferr[j - 1] = ferr[j - 1] / lstres;
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -564,6 +777,11 @@ double oracle_dget02(char trans, int m, int n, int nrhs, const double* a, int ld
 }
 
 // DGET08: the infinity-norm form of DGET02 (TESTING/LIN/dget08.f:178-213).
+int oracle_dgerfs(char trans, int n, int nrhs, const double* a, int lda, const double* af, int ldaf,
+                  const int* ipiv, const double* b, int ldb, double* x, int ldx, double* ferr, double* berr) {
+  return gerfs(trans, n, nrhs, a, lda, af, ldaf, ipiv, b, ldb, x, ldx, ferr, berr);
+}
+
 double oracle_dget08(char trans, int m, int n, int nrhs, const double* a, int lda, const double* x, int ldx,
                      const double* b, int ldb) {
   if (m <= 0 || n <= 0 || nrhs == 0) return 0.0;

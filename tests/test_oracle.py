@@ -343,3 +343,42 @@ def test_oracle_reproduces_reference_golden_n1024():
     assert info == int(g["info"]) == 0
     assert np.array_equal(ipiv, g["ipiv"])
     assert np.array_equal(x[:, 0].view(np.uint64), g["x"].view(np.uint64))
+
+
+@ref
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (17, 2), (120, 3)])
+def test_dgerfs_bitwise_vs_reference(trans, n, nrhs):
+    """DGERFS (refined X, FERR, BERR) of the restatement equals the reference
+    build bit for bit, starting from an FP32-accurate solution so that the
+    refinement loop and DLACN2's iterations all run."""
+    a = la.matrix(2, 3000 + n, n, n)
+    b = la.matrix(2, 3001 + n, n, nrhs)
+    lu, p, info = oracle.dgetrf(a)
+    assert info == 0
+    x0, _ = oracle.dgetrs(lu, p, b, trans)
+    x0 = x0.astype(np.float32).astype(np.float64)
+    r1 = oracle.dgerfs(a, lu, p, b, x0, trans)
+    r2 = oracle.ref_dgerfs(a, lu, p, b, x0, trans)
+    assert r1[3] == r2[3] == 0
+    for u, v in zip(r1[:3], r2[:3]):
+        assert np.array_equal(bits(u), bits(v))
+
+
+def test_dgerfs_bounds():
+    """DGET07's checks (TESTING/LIN/dget07.f): BERR below (n+1) eps-scale and
+    FERR bounding the true forward error of a known solution."""
+    n, nrhs = 80, 2
+    a = la.matrix(2, 41, n, n)
+    xact = la.matrix(2, 42, n, nrhs)
+    b = np.asfortranarray(a @ xact)
+    lu, p, _ = oracle.dgetrf(a)
+    x0, _ = oracle.dgetrs(lu, p, b)
+    x, ferr, berr, info = oracle.dgerfs(a, lu, p, b, x0)
+    assert info == 0
+    eps = 2.0 ** -53
+    assert np.all(berr <= (n + 1) * eps * 30)
+    err = np.max(np.abs(x - xact), axis=0) / np.max(np.abs(x), axis=0)
+    # the bound of the computed problem holds up to the O(eps) perturbation b carries
+    assert np.all(err <= ferr + 30 * eps * np.linalg.cond(a, np.inf))
+    assert np.all(ferr < 1e-10)
