@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--mp-n", type=int, default=32768, help="order of the mixed-precision comparison")
     p.add_argument("--no-mp", action="store_true", help="skip the mixed-precision comparison")
+    p.add_argument("--mp-half", action="store_true",
+                   help="add the FP16 / BF16 GMRES-IR variants to the mixed-precision comparison")
     p.add_argument("--calib-copy-mib", type=int, default=0,
                    help="run one plain device copy of this many MiB first (calibrates HBM byte counters)")
     return p.parse_args()
@@ -70,11 +72,12 @@ def cpu_baseline(n_sample, seed):
     }
 
 
-def mp_compare(n, seed):
+def mp_compare(n, seed, half=False):
     """Time-to-FP64-backward-error at order n (A, b ~ N(0,1), nrhs = 1): the plain
     FP64 DGESV path against the mixed-precision solvers (DSGESV: FP32 LU + FP64
-    refinement; DGESV_IR with FP16 / BF16 trailing updates after DGEEQUB, FP64
-    refinement and GMRES-IR).  Each variant reports its time, ITER, its normwise
+    refinement; DGESV_IR with an FP32 LU; with half=True also DGESV_IR with
+    FP16 / BF16 trailing updates after DGEEQUB, FP64 refinement and GMRES-IR,
+    which on N(0,1) at this order needs hundreds of GMRES steps).  Each variant reports its time, ITER, its normwise
     backward error max|b - A x| / (||A||_inf max|x|) computed in FP64 on the GPU,
     and its forward error against the FP64 path's solution."""
     import numpy as np
@@ -114,10 +117,12 @@ def mp_compare(n, seed):
         best = ms if best is None else min(best, ms)
     res["fp64_dgesv"] = {"ms": round(best, 2), "berr": berr(x64)}
     xn = x64.abs().max().item()
-    variants = [("fp32_dsgesv", None), ("fp32_ir", "S"), ("fp16_ir", "H"), ("bf16_ir", "B")]
+    variants = [("fp32_dsgesv", None), ("fp32_ir", "S")]
+    if half:
+        variants += [("fp16_ir", "H"), ("bf16_ir", "B")]
     for name, prec in variants:
         best, out = None, None
-        for _ in range(2):
+        for _ in range(2 if prec in (None, "S") else 1):
             d_a.copy_(d_a0)
             x = torch.zeros_like(d_b)
             if prec is None:
@@ -166,11 +171,16 @@ def main():
     d_info = torch.zeros(nst, dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
 
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(nst)]
+
     def step(i):
+        evs[i][0].record(stream)
         rc = la.dgetrf_async(n, n, d_as[i], n, d_ipiv[i], d_info[i:i + 1], stream)
         assert rc == 0, rc
+        evs[i][1].record(stream)
         rc = la.dgetrs_async("N", n, nrhs, d_as[i], n, d_ipiv[i], d_bs[i], n, stream)
         assert rc == 0, rc
+        evs[i][2].record(stream)
 
     if args.calib_copy_mib > 0:
         src = torch.empty(args.calib_copy_mib << 17, dtype=torch.float64, device=dev).fill_(1.0)
@@ -211,7 +221,9 @@ def main():
     ratio = (r.abs().sum(dim=1) / (anorm * x.abs().sum(dim=1) * 2.0 ** -53)).max().item()
 
     gemm_ms, gemm_calls, gemm_flops = ph["gemm"]
-    getrf_ms = sum(ph[p][0] for p in ("transpose", "panel", "laswp", "trsm", "gemm_panel", "gemm"))
+    # DGETRF and DGETRS times of the timed steps (events on the caller's stream)
+    getrf_ms = sum(evs[i][0].elapsed_time(evs[i][1]) for i in range(K))
+    getrs_ms = sum(evs[i][1].elapsed_time(evs[i][2]) for i in range(K))
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     out = {
         "metric": METRIC,
@@ -232,7 +244,8 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": "gemm_kernel<double,N,N,update> (DGETRF trailing update A22 -= A21*U12)",
+            "kernel": ("DGETRF trailing update A22 -= A21*U12: update_big_kernel<double,4,2> (256x128 tiles), "
+                       "gemm_kernel<double,N,N,update> for the next panel's columns"),
             "achieved": round(achieved, 3),
             "peak": PEAK_FP64,
             "unit": "TFLOP/s",
@@ -243,13 +256,15 @@ def main():
         },
         "getrf_tflops": round(K * f_getrf / (getrf_ms * 1e-3) / 1e12, 3) if getrf_ms > 0 else None,
         "getrf_frac_of_fp64_peak": round(K * f_getrf / (getrf_ms * 1e-3) / 1e12 / PEAK_FP64, 4) if getrf_ms > 0 else None,
+        "getrf_ms_per_step": round(getrf_ms / K, 3),
+        "getrs_ms_per_step": round(getrs_ms / K, 3),
         "phase_ms_per_step": {p: round(ph[p][0] / K, 3) for p in ph},
         "dget02_ratio": round(ratio, 4),
     }
     if not args.no_mp:
         del d_as, d_bs, d_a0
         torch.cuda.empty_cache()
-        out["mp_ir"] = mp_compare(args.mp_n, 32768)
+        out["mp_ir"] = mp_compare(args.mp_n, 32768, args.mp_half)
     if not args.no_cpu:
         print(f"[bench] CPU baseline sample n={args.cpu_n}", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.seed)

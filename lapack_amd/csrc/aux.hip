@@ -59,22 +59,111 @@ void launch_transpose_t(int rows, int cols, const Ti* in, long ld_in, To* out, l
 constexpr int SWP_ROWS = 16;      // T rows per workgroup
 constexpr int SWP_MAXPAIRS = 512; // 2 * the largest panel width
 
+// Composition of one block of K <= 256 interchanges in parallel.  Steps are
+// positions q = b0 .. b0+K-1, step q swapping q with its target p_q >= q
+// (the form DGETRF's IPIV always has).  Write c_q for the content of
+// position q just before step q.  Before step q position q was last written
+// by the latest earlier step u with p_u = q, which left c_u there, so
+//   c_q = c_par(q), par(q) = max{u < q : p_u = q}, or q itself if none --
+// a forest whose roots are their own contents, collapsed by pointer jumping.
+// Position q is final after step q and holds what p_q held before it: c_q
+// when p_q = q, else c_prv(q), prv(q) = max{u < q : p_u = p_q}, or p_q itself
+// if none.  A target past the block holds c_u of the last step u aiming at
+// it.  Every touched position gets exactly one (dst, src) pair.  Called by
+// all 256 threads; s_p: the K targets (absolute, 0-based); s_r: 256 ints of
+// scratch; pairs go to s_out through the counter *s_cnt.
+__device__ void compose_block(int b0, int K, const int* s_p, int* s_r, int* s_out, int* s_cnt) {
+  const int t = threadIdx.x;
+  const int q = b0 + t;
+  int p = q, par = -1, prv = -1;
+  bool last = true;
+  if (t < K) {
+    p = s_p[t];
+    // s_p holds -1 past K up to a multiple of 4: never a position
+    const int Kr = (K + 3) & ~3;
+#pragma unroll 4
+    for (int u0 = 0; u0 < Kr; u0 += 4) {
+      const int4 pv = *reinterpret_cast<const int4*>(&s_p[u0]);
+      const int pu[4] = {pv.x, pv.y, pv.z, pv.w};
+#pragma unroll
+      for (int z = 0; z < 4; ++z) {
+        const int u = u0 + z;
+        if (u < t) {
+          if (pu[z] == q) par = u;
+          if (pu[z] == p) prv = u;
+        } else if (u > t && pu[z] == p) {
+          last = false;
+        }
+      }
+    }
+    s_r[t] = par >= 0 ? par : t;
+  }
+  __syncthreads();
+  // pointer jumping until every chain has reached its root (at most 8
+  // rounds for 256 steps; usually one)
+  for (int it = 0; it < 8; ++it) {
+    const int cur = t < K ? s_r[t] : 0;
+    const int nr = t < K ? s_r[cur] : 0;
+    const int moved = __syncthreads_or(nr != cur ? 1 : 0);
+    if (!moved) break;
+    if (t < K) s_r[t] = nr;
+    __syncthreads();
+  }
+  if (t < K) {
+    const int src = p == q ? b0 + s_r[t] : (prv >= 0 ? b0 + s_r[prv] : p);
+    if (src != q) {
+      const int o = atomicAdd(s_cnt, 1);
+      s_out[2 * o] = q;
+      s_out[2 * o + 1] = src;
+    }
+    if (last && p >= b0 + K) {
+      const int o = atomicAdd(s_cnt, 1);
+      s_out[2 * o] = p;
+      s_out[2 * o + 1] = b0 + s_r[t];
+    }
+  }
+  __syncthreads();
+}
+
 // pairs[2q] = dst, pairs[2q+1] = src (0-based positions); *npairs = count.
-// Positions live in [k1, npos); map is LDS (or global when it does not fit).
+// Positions live in [k1, npos).  Up to 256 interchanges that all satisfy
+// p >= q are composed in parallel (compose_block); otherwise one thread walks
+// the sequence over a position map (LDS, or global when it does not fit).
 template <bool LDSMAP>
 __global__ __launch_bounds__(256) void swaps_to_pairs_kernel(int k1, int k2, int npos, const int* __restrict__ ipiv,
                                                              int* __restrict__ pairs, int* __restrict__ npairs,
                                                              int* __restrict__ gmap) {
   extern __shared__ int smem[];
   __shared__ int s_cnt;
+  __shared__ __align__(16) int s_ip[SWP_MAXPAIRS / 2];
+  __shared__ int s_r[256];
+  __shared__ int s_out[2 * SWP_MAXPAIRS];
+  const int K = k2 - k1;
+  if ((int)threadIdx.x >= K && threadIdx.x < SWP_MAXPAIRS / 2) s_ip[threadIdx.x] = -1;
+  // the sequence itself is read once, in parallel; entries outside [k1, npos)
+  // are ignored, as a self-interchange
+  bool bad = false;
+  for (int x = threadIdx.x; x < K && x < SWP_MAXPAIRS / 2; x += 256) {
+    int p = ipiv[k1 + x] - 1;
+    if (p < k1 || p >= npos) p = k1 + x;
+    if (p < k1 + x) bad = true;
+    s_ip[x] = p;
+  }
+  if (threadIdx.x == 0) s_cnt = 0;
+  if (__syncthreads_or(bad ? 1 : 0) == 0 && K <= 256) {
+    compose_block(k1, K, s_ip, s_r, s_out, &s_cnt);
+    const int np = min(s_cnt, SWP_MAXPAIRS);
+    for (int x = threadIdx.x; x < 2 * np; x += 256) pairs[x] = s_out[x];
+    if (threadIdx.x == 0) *npairs = np;
+    return;
+  }
   int* cur = LDSMAP ? smem : gmap;
   const int span = npos - k1;
   for (int x = threadIdx.x; x < span; x += 256) cur[x] = k1 + x;
-  if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = k1; i < k2; ++i) {
-      const int p = ipiv[i] - 1;
+      const int p = i - k1 < SWP_MAXPAIRS / 2 ? s_ip[i - k1] : ipiv[i] - 1;
       if (p != i && p >= k1 && p < npos) {
         const int t = cur[i - k1];
         cur[i - k1] = cur[p - k1];
@@ -100,12 +189,19 @@ __global__ __launch_bounds__(256) void swaps_to_pairs_kernel(int k1, int k2, int
 template <class T>
 __global__ __launch_bounds__(256) void apply_pairs_cols_kernel(T* __restrict__ Tm, long ldt, int r0, int r1, int r2,
                                                                int r3, const int* __restrict__ pairs,
-                                                               const int* __restrict__ npairs) {
+                                                               const int* __restrict__ npairs, int nblk,
+                                                               int skip0) {
   __shared__ T st[SWP_MAXPAIRS * SWP_ROWS];
+  // skip0: blocks dispatched to XCD 0 (blockIdx % 8 == 0) leave at once
+  int blk = blockIdx.x;
+  if (skip0) {
+    if (blk % 8 == 0) return;
+    blk = (blk / 8) * 7 + (blk % 8 - 1);
+  }
+  if (blk >= nblk) return;
   const int np = *npairs;
   // this block's 16 rows: first from [r0, r1), then from [r2, r3)
   const int n1 = (r1 - r0 + SWP_ROWS - 1) / SWP_ROWS;
-  const int blk = blockIdx.x;
   const long c0 = blk < n1 ? (long)r0 + (long)blk * SWP_ROWS : (long)r2 + (long)(blk - n1) * SWP_ROWS;
   const long cend = blk < n1 ? r1 : r3;
   const int cl = threadIdx.x & (SWP_ROWS - 1);
@@ -117,6 +213,108 @@ __global__ __launch_bounds__(256) void apply_pairs_cols_kernel(T* __restrict__ T
   __syncthreads();
   for (int q = qg; q < np; q += 256 / SWP_ROWS)
     if (ok) Tm[c + (long)pairs[2 * q] * ldt] = st[q * SWP_ROWS + cl];
+}
+
+// Everything between two leaves of a panel in one single-workgroup launch
+// (a single workgroup never waits for a share of CUs held by a concurrent
+// trailing update, which a many-workgroup launch does):
+//   1. the leaf's interchanges ipiv[jj..jj+w) (targets in [jj, m), each at
+//      or after its step, as the panel kernels produce them) composed into
+//      pairs (compose_block);
+//   2. the pairs applied to the panel's other T rows, [j, jj) and
+//      [jj+w, jend) (SRC/dgetrf2.f:234-236 on the columns left and right of
+//      the leaf), 64 T rows at a time through registers;
+//   3. the in-panel DTRSM of SRC/dgetrf2.f:240-241 on the columns right of
+//      the leaf: each is solved against the leaf's unit lower triangle by one
+//      thread in the reference's order (BLAS/SRC/dtrsm.f:177-190: for k, skip
+//      a zero x_k, else x_i -= x_k L(i,k) for i > k).
+template <class T>
+__global__ __launch_bounds__(256) void panel_boundary_kernel(T* __restrict__ Tm, long ldt, int j, int jj, int w,
+                                                             int jend, int m, const int* __restrict__ ipiv) {
+  constexpr int W = 64;
+  __shared__ __align__(16) int s_p[256];
+  __shared__ int s_r[256];
+  __shared__ int s_out[2 * SWP_MAXPAIRS];
+  __shared__ int s_cnt;
+  __shared__ T sL[W][W + 1];
+  const int tid = threadIdx.x;
+  s_p[tid] = -1;
+  if (tid < w) {
+    int p = ipiv[jj + tid] - 1;
+    if (p < jj + tid || p >= m) p = jj + tid;  // not of the panel's form: left alone
+    s_p[tid] = p;
+  }
+  if (tid == 0) s_cnt = 0;
+  __syncthreads();
+  compose_block(jj, w, s_p, s_r, s_out, &s_cnt);
+  const int np = min(s_cnt, 2 * W);
+  const int n1 = jj - j, rest = jend - jj - w;
+  const int tot = n1 + rest;
+  // The leaf's triangle (rows [jj, jj+w) are not interchanged here) is read
+  // together with the first chunk of the interchange: L(i,k) = A(jj+i, jj+k)
+  // = T(jj+k, jj+i) -> sL[k][i], zero outside the leaf.
+  T lv[W * W / 256];
+  if (rest > 0) {
+#pragma unroll
+    for (int u = 0; u < W * W / 256; ++u) {
+      const int e = tid + 256 * u;
+      const int k = e % W, i = e / W;
+      lv[u] = (k < w && i < w) ? Tm[(long)(jj + k) + (long)(jj + i) * ldt] : T(0);
+    }
+  }
+  // 64 T rows at a time: every source value of the chunk in registers
+  // (32 per thread), one barrier, every destination (chunks touch disjoint
+  // rows, so the next chunk's loads need no barrier after these stores)
+  constexpr int CH = 64;
+  constexpr int PER = 2 * W * CH / 256;
+  for (int r0 = 0; r0 < tot; r0 += CH) {
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + 256 * u;
+      const int rr = e % CH, q = e / CH;
+      const int ri = r0 + rr;
+      v[u] = T(0);
+      if (q < np && ri < tot) {
+        const long trow = ri < n1 ? (long)j + ri : (long)jj + w + (ri - n1);
+        v[u] = Tm[trow + (long)s_out[2 * q + 1] * ldt];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + 256 * u;
+      const int rr = e % CH, q = e / CH;
+      const int ri = r0 + rr;
+      if (q < np && ri < tot) {
+        const long trow = ri < n1 ? (long)j + ri : (long)jj + w + (ri - n1);
+        Tm[trow + (long)s_out[2 * q] * ldt] = v[u];
+      }
+    }
+  }
+  if (rest <= 0) return;
+#pragma unroll
+  for (int u = 0; u < W * W / 256; ++u) {
+    const int e = tid + 256 * u;
+    sL[e % W][e / W] = lv[u];
+  }
+  __syncthreads();
+  if (tid < rest) {
+    const long trow = (long)jj + w + tid;
+    T x[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) x[i] = i < w ? Tm[trow + (long)(jj + i) * ldt] : T(0);
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      if (x[k] != T(0)) {
+#pragma unroll
+        for (int i = k + 1; i < W; ++i) x[i] = x[i] - x[k] * sL[k][i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < W; ++i)
+      if (i < w) Tm[trow + (long)(jj + i) * ldt] = x[i];
+  }
 }
 
 // Row interchanges of a column-major matrix through the same pairs:
@@ -182,6 +380,66 @@ __global__ __launch_bounds__(256) void swaps_to_perm_kernel(int nrows, int k1, i
     for (int r = threadIdx.x; r < nrows; r += 256) perm[r] = cur[r];
 }
 
+// The same permutation for the sequences DGETRF produces (every target at or
+// after its step), in blocks of 256 interchanges: each block is composed in
+// parallel (compose_block) and applied to the position map as one gather;
+// a block breaking that form is walked serially.  INCX < 0 applies the
+// interchanges in reverse order, i.e. the inverse permutation of the
+// forward sequence: the forward map is built and inverted.
+__global__ __launch_bounds__(256) void swaps_to_perm_blocked_kernel(int nrows, int k1, int k2,
+                                                                    const int* __restrict__ ipiv, int incx,
+                                                                    int* __restrict__ perm) {
+  extern __shared__ int cur[];  // nrows ints
+  __shared__ __align__(16) int s_p[256];
+  __shared__ int s_r[256];
+  __shared__ int s_out[2 * SWP_MAXPAIRS];
+  __shared__ int s_cnt;
+  for (int r = threadIdx.x; r < nrows; r += 256) cur[r] = r;
+  for (int b0 = k1; b0 < k2; b0 += 256) {
+    const int K = min(256, k2 - b0);
+    bool bad = false;
+    s_p[threadIdx.x] = -1;
+    if ((int)threadIdx.x < K) {
+      const int q = b0 + threadIdx.x;
+      int p = ipiv[q] - 1;
+      if (p < 0 || p >= nrows) p = q;  // ignored, as a self-interchange
+      if (p < q) bad = true;
+      s_p[threadIdx.x] = p;
+    }
+    if (threadIdx.x == 0) s_cnt = 0;
+    if (__syncthreads_or(bad ? 1 : 0) == 0) {
+      compose_block(b0, K, s_p, s_r, s_out, &s_cnt);
+      const int np = s_cnt;
+      int d[2], v[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int o = threadIdx.x + 256 * h;
+        d[h] = o < np ? s_out[2 * o] : -1;
+        v[h] = o < np ? cur[s_out[2 * o + 1]] : 0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (d[h] >= 0) cur[d[h]] = v[h];
+    } else if (threadIdx.x == 0) {
+      for (int t = 0; t < K; ++t) {
+        const int i = b0 + t, p = s_p[t];
+        if (p != i) {
+          const int a = cur[i];
+          cur[i] = cur[p];
+          cur[p] = a;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (incx > 0) {
+    for (int r = threadIdx.x; r < nrows; r += 256) perm[r] = cur[r];
+  } else {
+    for (int r = threadIdx.x; r < nrows; r += 256) perm[cur[r]] = r;
+  }
+}
+
 // dst(r,j) = src(perm[r], j) for r < nrows, j < ncols.
 template <class T>
 __global__ __launch_bounds__(256) void gather_rows_kernel(int nrows, int ncols, const int* __restrict__ perm,
@@ -243,9 +501,16 @@ __global__ __launch_bounds__(256) void trtri_kernel(int n, int nb, const T* __re
 // ---- copies and fills -----------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void copy_kernel(int rows, int cols, const T* __restrict__ src, long lds,
-                                                   T* __restrict__ dst, long ldd) {
+                                                   T* __restrict__ dst, long ldd, int skip0) {
+  // skip0: blocks dispatched to XCD 0 (blockIdx % 8 == 0) leave at once
+  long lb = blockIdx.x, ng = gridDim.x;
+  if (skip0) {
+    if (blockIdx.x % 8 == 0) return;
+    lb = (long)(blockIdx.x / 8) * 7 + (blockIdx.x % 8 - 1);
+    ng = (long)(gridDim.x / 8) * 7;
+  }
   const long tot = (long)rows * cols;
-  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
+  for (long e = lb * 256 + threadIdx.x; e < tot; e += ng * 256) {
     const long i = e % rows, j = e / rows;
     dst[i + j * ldd] = src[i + j * lds];
   }
@@ -299,7 +564,7 @@ int swp_max_pairs() { return SWP_MAXPAIRS; }
 // [k1, npos); gmap: npos ints of scratch used when the map does not fit LDS.
 void launch_swaps_to_pairs(int k1, int k2, int npos, const int* ipiv, int* pairs, int* npairs, int* gmap,
                            hipStream_t s) {
-  const size_t need = sizeof(int) * (size_t)(npos - k1) + 64;
+  const size_t need = sizeof(int) * (size_t)(npos - k1) + 8192;  // + the kernel's static LDS
   if (need <= (size_t)lds_limit())
     swaps_to_pairs_kernel<true><<<1, 256, sizeof(int) * (size_t)(npos - k1), s>>>(k1, k2, npos, ipiv, pairs, npairs, gmap);
   else
@@ -307,19 +572,29 @@ void launch_swaps_to_pairs(int k1, int k2, int npos, const int* ipiv, int* pairs
 }
 template <class T>
 static void apply_cols_t(T* Tm, long ldt, int r0, int r1, int r2, int r3, const int* pairs, const int* npairs,
-                         hipStream_t s) {
+                         hipStream_t s, bool skip0) {
   if (r1 < r0) r1 = r0;
   if (r3 < r2) r3 = r2;
   const int nb = (r1 - r0 + SWP_ROWS - 1) / SWP_ROWS + (r3 - r2 + SWP_ROWS - 1) / SWP_ROWS;
-  if (nb > 0) apply_pairs_cols_kernel<T><<<nb, 256, 0, s>>>(Tm, ldt, r0, r1, r2, r3, pairs, npairs);
+  const int grid = skip0 ? (nb + 6) / 7 * 8 : nb;
+  if (nb > 0)
+    apply_pairs_cols_kernel<T><<<grid, 256, 0, s>>>(Tm, ldt, r0, r1, r2, r3, pairs, npairs, nb, skip0 ? 1 : 0);
 }
 void launch_apply_pairs_cols_d(double* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs,
-                               const int* npairs, hipStream_t s) {
-  apply_cols_t<double>(T, ldt, r0, r1, r2, r3, pairs, npairs, s);
+                               const int* npairs, hipStream_t s, bool skip0) {
+  apply_cols_t<double>(T, ldt, r0, r1, r2, r3, pairs, npairs, s, skip0);
 }
 void launch_apply_pairs_cols_s(float* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs,
-                               const int* npairs, hipStream_t s) {
-  apply_cols_t<float>(T, ldt, r0, r1, r2, r3, pairs, npairs, s);
+                               const int* npairs, hipStream_t s, bool skip0) {
+  apply_cols_t<float>(T, ldt, r0, r1, r2, r3, pairs, npairs, s, skip0);
+}
+void launch_panel_boundary_d(double* T, long ldt, int j, int jj, int w, int jend, int m, const int* ipiv,
+                             hipStream_t s) {
+  panel_boundary_kernel<double><<<1, 256, 0, s>>>(T, ldt, j, jj, w, jend, m, ipiv);
+}
+void launch_panel_boundary_s(float* T, long ldt, int j, int jj, int w, int jend, int m, const int* ipiv,
+                             hipStream_t s) {
+  panel_boundary_kernel<float><<<1, 256, 0, s>>>(T, ldt, j, jj, w, jend, m, ipiv);
 }
 void launch_apply_pairs_rows_d(double* B, long ldb, int ncols, const int* pairs, const int* npairs, hipStream_t s) {
   if (ncols > 0) apply_pairs_rows_kernel<double><<<(ncols + SWP_ROWS - 1) / SWP_ROWS, 256, 0, s>>>(B, ldb, ncols, pairs, npairs);
@@ -330,6 +605,10 @@ void launch_apply_pairs_rows_s(float* B, long ldb, int ncols, const int* pairs, 
 
 // perm (nrows ints) <- permutation of the interchange sequence; see above.
 void launch_swaps_to_perm(int nrows, int k1, int k2, const int* ipiv, int incx, int* perm, hipStream_t s) {
+  if (k1 >= 0 && k2 <= nrows && sizeof(int) * (size_t)nrows + 8192 <= (size_t)lds_limit()) {
+    swaps_to_perm_blocked_kernel<<<1, 256, sizeof(int) * (size_t)nrows, s>>>(nrows, k1, k2, ipiv, incx, perm);
+    return;
+  }
   const size_t need = sizeof(int) * (PERM_CHUNK + (size_t)nrows);
   if (nrows <= PERM_LDS_MAX && need <= (size_t)lds_limit()) {
     const size_t sh = sizeof(int) * (PERM_CHUNK + (size_t)nrows);
@@ -381,11 +660,19 @@ void launch_trtri_un_s(int nb, const float* U, long ldu, float* W, long ldw, hip
   launch_trtri_t<float>(true, false, nb, nb, U, ldu, W, ldw, s);
 }
 
-void launch_copy_d(int rows, int cols, const double* src, long lds, double* dst, long ldd, hipStream_t s) {
-  if (rows > 0 && cols > 0) copy_kernel<double><<<grid_for((long)rows * cols), 256, 0, s>>>(rows, cols, src, lds, dst, ldd);
+static int copy_grid(long work, bool skip0) {
+  const int g = grid_for(work);
+  return skip0 ? (g + 6) / 7 * 8 : g;
 }
-void launch_copy_s(int rows, int cols, const float* src, long lds, float* dst, long ldd, hipStream_t s) {
-  if (rows > 0 && cols > 0) copy_kernel<float><<<grid_for((long)rows * cols), 256, 0, s>>>(rows, cols, src, lds, dst, ldd);
+void launch_copy_d(int rows, int cols, const double* src, long lds, double* dst, long ldd, hipStream_t s,
+                   bool skip0) {
+  if (rows > 0 && cols > 0)
+    copy_kernel<double><<<copy_grid((long)rows * cols, skip0), 256, 0, s>>>(rows, cols, src, lds, dst, ldd, skip0);
+}
+void launch_copy_s(int rows, int cols, const float* src, long lds, float* dst, long ldd, hipStream_t s,
+                   bool skip0) {
+  if (rows > 0 && cols > 0)
+    copy_kernel<float><<<copy_grid((long)rows * cols, skip0), 256, 0, s>>>(rows, cols, src, lds, dst, ldd, skip0);
 }
 void launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s) {
   if (n) fill_kernel<unsigned><<<grid_for((long)n), 256, 0, s>>>(p, n, v);

@@ -105,14 +105,17 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
       for (int r = tid; r < mr; r += NT)
         for (int c = 0; c < w; ++c) P[c * ldp + r] = A[(k0 + r) + (long)(k0 + c) * lda];
       __syncthreads();
-      // ---- panel in LDS
+      // ---- panel in LDS.  Per column: the keys of column j were taken at
+      // the end of the previous column's update (or just below for j = 0);
+      // B1 -> thread 0 decides -> B2 -> interchange -> B3 -> scale and
+      // update (each thread its own rows) fused with the next column's keys.
+      u64 key = 0;
+      int row = 0x7fffffff;
+      for (int r = tid; r < mr; r += NT) {
+        const u64 kk = bkey(P[r], r == 0);
+        if (kk > key) { key = kk; row = r; }
+      }
       for (int j = 0; j < w; ++j) {
-        u64 key = 0;
-        int row = 0x7fffffff;
-        for (int r = j + tid; r < mr; r += NT) {
-          const u64 kk = bkey(P[j * ldp + r], r == j);
-          if (kk > key) { key = kk; row = r; }
-        }
         wave_argmax(key, row);
         if ((tid & 63) == 0) { s_key[tid >> 6] = key; s_row[tid >> 6] = row; }
         __syncthreads();
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
           int br = s_row[0];
           for (int q = 1; q < NW; ++q)
             if (s_key[q] > bk || (s_key[q] == bk && s_row[q] < br)) { bk = s_key[q]; br = s_row[q]; }
-          if (br >= mr) br = j;  // cannot happen (row j is always a candidate); keeps indexes in range
+          if (br >= mr || br < j) br = j;  // cannot happen (row j is always a candidate); keeps indexes in range
           s_piv[j] = br;
           s_pv = P[j * ldp + br];
           ipiv[k0 + j] = k0 + br + 1;
@@ -133,25 +136,29 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
         // have read it.
         const int p = s_piv[j];
         const T pv = s_pv;
-        if (pv != T(0)) {
-          if (p != j)
-            for (int c = tid; c < w; c += NT) {
-              const T t = P[c * ldp + j];
-              P[c * ldp + j] = P[c * ldp + p];
-              P[c * ldp + p] = t;
-            }
-          __syncthreads();
-          if (fabs(pv) >= sfmin) {
-            const T rcp = T(1) / pv;
-            for (int r = j + 1 + tid; r < mr; r += NT) P[j * ldp + r] = rcp * P[j * ldp + r];
-          } else {
-            for (int r = j + 1 + tid; r < mr; r += NT) P[j * ldp + r] = P[j * ldp + r] / pv;
+        if (pv != T(0) && p != j) {
+          for (int c = tid; c < w; c += NT) {
+            const T t = P[c * ldp + j];
+            P[c * ldp + j] = P[c * ldp + p];
+            P[c * ldp + p] = t;
           }
+          __syncthreads();
         }
-        __syncthreads();
+        const bool recip = fabs(pv) >= sfmin;
+        const T rcp = T(1) / pv;
+        key = 0;
+        row = 0x7fffffff;
         for (int r = j + 1 + tid; r < mr; r += NT) {
-          const T l = P[j * ldp + r];
+          T l = P[j * ldp + r];
+          if (pv != T(0)) {
+            l = recip ? rcp * l : l / pv;
+            P[j * ldp + r] = l;
+          }
           for (int c = j + 1; c < w; ++c) P[c * ldp + r] = P[c * ldp + r] - l * P[c * ldp + j];
+          if (j + 1 < w) {
+            const u64 kk = bkey(P[(j + 1) * ldp + r], r == j + 1);
+            if (kk > key) { key = kk; row = r; }
+          }
         }
         __syncthreads();
       }

@@ -11,15 +11,19 @@
 //     i.e. for workgroups already running, so every wait ends;
 //   * thread (i, g) owns row i of the block and the columns c in
 //     [16g, 16g+16) of every 64-column tile of M it depends on; tiles are
-//     read straight from HBM, four dependency blocks per round: their
+//     read straight from HBM, up to two dependency blocks per round: their
 //     flags are polled together, their x blocks staged in LDS together, and
-//     the thread accumulates acc(q) -= sum_c M(i,c) x(c,q) for all q;
-//   * the four partial sums per row are added in LDS, y = b + acc, and the
-//     diagonal block is solved by substitution, one wavefront per four
-//     right-hand sides, lane i holding row i: for each k, x_k is finished
-//     (divided by the diagonal when M is not unit) and broadcast, and the
-//     rows below (forward) or above (backward) subtract M(i,k) x_k -- the
-//     column-oriented order of BLAS/SRC/dtrsm.f:277-300, with its zero test;
+//     the round's products are summed on their own before they join the
+//     running sum (two-level summation: no sum runs over more than 64 terms
+//     of one round or one term per round).  The block's last dependency --
+//     the one finished most recently -- gets a round of its own, so the
+//     work left once it arrives is one tile;
+//   * before its dependencies arrive, the workgroup inverts the 64x64
+//     diagonal block in LDS (doubling over 2x2 block-triangular inverses),
+//     so that once they have arrived the block costs a 64-term product per
+//     row, not a 64-step chain;
+//   * the four partial sums per row are added in LDS, y = b + acc, and
+//     x = inv(M_dd) y, one wavefront per four right-hand sides;
 //   * x is stored with device-coherent stores, the stores are waited for,
 //     and the block's flag is raised.
 #include "la_common.h"
@@ -28,20 +32,24 @@ namespace la {
 namespace {
 
 constexpr int TB = 64;    // rows per block
-constexpr int TQ = 16;    // right-hand sides per launch
-constexpr int DEPB = 4;   // dependency blocks per round
+constexpr int TQ = 16;    // right-hand sides per launch (at most)
+constexpr int DEPB = 2;   // dependency blocks per round
 
 template <bool TRANS>
 __device__ __forceinline__ long m_index(int i, int c, long lda) {
   return TRANS ? (long)c + (long)i * lda : (long)i + (long)c * lda;
 }
 
-template <class T, bool LOWERM, bool UNIT, bool TRANS>
+// QN: right-hand sides handled per launch (1, 4 or 16); QW of them per wave.
+template <class T, int QN, bool LOWERM, bool UNIT, bool TRANS>
 __global__ __launch_bounds__(256) void tri_chain_kernel(int n, int nrhs, const T* __restrict__ A, long lda,
                                                         T* __restrict__ X, long ldx, unsigned* ticket,
                                                         unsigned* flags, int* err) {
-  __shared__ T sD[TB][TB + 1];             // diagonal block of M
-  __shared__ T sx[DEPB][TB][TQ];           // x blocks of a round; reused for the partial sums
+  constexpr int QW = QN < 4 ? QN : 4;
+  constexpr int NWQ = QN / QW;  // waves taking part in the diagonal solve
+  __shared__ T sD[TB][TB + 1];             // diagonal block of M (rows past n: identity)
+  constexpr int QS = QN < 4 ? 4 : QN;      // >= 1024 values: the inversion's temporary
+  __shared__ T sx[4][TB][QS];              // x blocks of a round; reused for the partial sums
   __shared__ int s_t;
   const int tid = threadIdx.x;
   const int nblk = (n + TB - 1) / TB;
@@ -53,21 +61,82 @@ __global__ __launch_bounds__(256) void tri_chain_kernel(int n, int nrhs, const T
   const int r0 = bi * TB;
   const int bs = min(TB, n - r0);
 
-  for (int e = tid; e < TB * TB; e += 256) {
-    int i, c;
-    if (TRANS) { c = e % TB; i = e / TB; } else { i = e % TB; c = e / TB; }
-    sD[i][c] = (i < bs && c < bs) ? A[m_index<TRANS>(r0 + i, r0 + c, lda)] : T(0);
+  {
+    T dv[TB * TB / 256];
+#pragma unroll
+    for (int it = 0; it < TB * TB / 256; ++it) {
+      const int e = tid + it * 256;
+      int i, c;
+      if (TRANS) { c = e % TB; i = e / TB; } else { i = e % TB; c = e / TB; }
+      // M's own triangle only (the other holds the other factor); rows and
+      // columns past the matrix: identity
+      const bool tri = LOWERM ? c <= i : c >= i;
+      dv[it] = (i < bs && c < bs) ? (tri ? A[m_index<TRANS>(r0 + i, r0 + c, lda)] : T(0)) : (i == c ? T(1) : T(0));
+    }
+#pragma unroll
+    for (int it = 0; it < TB * TB / 256; ++it) {
+      const int e = tid + it * 256;
+      int i, c;
+      if (TRANS) { c = e % TB; i = e / TB; } else { i = e % TB; c = e / TB; }
+      sD[i][c] = dv[it];
+    }
+  }
+  __syncthreads();
+  // The diagonal block is replaced by its inverse while the workgroup would
+  // otherwise wait for its dependencies, by doubling: the 1x1 blocks are
+  // the reciprocals of the diagonal (1 when M is unit); then for b = 1, 2,
+  // .., 32 every pair of inverted b x b diagonal blocks A (first) and B
+  // (second) gets its off-diagonal block, -inv(B) C inv(A) below the
+  // diagonal (lower M) or -inv(A) C inv(B) above it (upper M), through a
+  // b x b temporary in the x staging area.  Once the dependencies have
+  // arrived the block costs a 64-term product per row, not a 64-step chain.
+  {
+    T* tmp = &sx[0][0][0];  // >= 1024 values
+    if (tid < TB) sD[tid][tid] = UNIT ? T(1) : T(1) / sD[tid][tid];
+    __syncthreads();
+    for (int bsz = 1; bsz < TB; bsz *= 2) {
+      const int cnt = (TB / (2 * bsz)) * bsz * bsz;  // elements of all pairs' blocks
+      // phase 1: tmp = C inv(A) (lower) or inv(A) C (upper), per pair
+      for (int e = tid; e < cnt; e += 256) {
+        const int pr = e / (bsz * bsz), r = (e / bsz) % bsz, c = e % bsz;
+        const int a0 = 2 * pr * bsz, b0 = a0 + bsz;
+        T acc = T(0);
+        if (LOWERM) {
+          for (int k = c; k < bsz; ++k) acc = acc + sD[b0 + r][a0 + k] * sD[a0 + k][a0 + c];
+        } else {
+          for (int k = r; k < bsz; ++k) acc = acc + sD[a0 + r][a0 + k] * sD[a0 + k][b0 + c];
+        }
+        tmp[e] = acc;
+      }
+      __syncthreads();
+      // phase 2: off-diagonal block = -inv(B) tmp (lower) or -tmp inv(B) (upper)
+      for (int e = tid; e < cnt; e += 256) {
+        const int pr = e / (bsz * bsz), r = (e / bsz) % bsz, c = e % bsz;
+        const int a0 = 2 * pr * bsz, b0 = a0 + bsz;
+        const T* tp = tmp + (size_t)pr * bsz * bsz;
+        T acc = T(0);
+        if (LOWERM) {
+          for (int k = 0; k <= r; ++k) acc = acc + sD[b0 + r][b0 + k] * tp[k * bsz + c];
+          sD[b0 + r][a0 + c] = -acc;
+        } else {
+          for (int k = 0; k <= c; ++k) acc = acc + tp[r * bsz + k] * sD[b0 + k][b0 + c];
+          sD[a0 + r][b0 + c] = -acc;
+        }
+      }
+      __syncthreads();
+    }
   }
 
   const int i = tid & (TB - 1);
   const int g = tid >> 6;
-  T acc[TQ];
+  T acc[QN];
 #pragma unroll
-  for (int q = 0; q < TQ; ++q) acc[q] = T(0);
+  for (int q = 0; q < QN; ++q) acc[q] = T(0);
   const int ndep = LOWERM ? bi : nblk - 1 - bi;
   bool failed = false;
-  for (int d0 = 0; d0 < ndep; d0 += DEPB) {
-    const int nd = min(DEPB, ndep - d0);
+  for (int d0 = 0; d0 < ndep;) {
+    // rounds of up to DEPB over all but the last dependency, then the last alone
+    const int nd = d0 < ndep - 1 ? min(DEPB, ndep - 1 - d0) : 1;
     // M values of this round, issued before the waits
     T mv[DEPB][16];
 #pragma unroll
@@ -90,76 +159,97 @@ __global__ __launch_bounds__(256) void tri_chain_kernel(int n, int nrhs, const T
       }
     }
     __syncthreads();
-    for (int e = tid; e < DEPB * TB * TQ; e += 256) {
-      const int c = e % TB, q = (e / TB) % TQ, u = e / (TB * TQ);
-      const int j = LOWERM ? d0 + u : nblk - 1 - (d0 + u);
-      T v = T(0);
-      if (u < nd && q < nrhs && j * TB + c < n) v = ld_dev(&X[(long)(j * TB + c) + (long)q * ldx]);
-      sx[u][c][q] = v;
-    }
-    __syncthreads();
+    {
+      // all of the round's x loads in flight at once, then into LDS
+      constexpr int PER = (DEPB * TB * QN + 255) / 256;
+      T xs[PER];
 #pragma unroll
-    for (int u = 0; u < DEPB; ++u) {
+      for (int it = 0; it < PER; ++it) {
+        const int e = tid + it * 256;
+        const int c = e % TB, q = (e / TB) % QN, u = e / (TB * QN);
+        const int j = LOWERM ? d0 + u : nblk - 1 - (d0 + u);
+        xs[it] = (u < nd && q < nrhs && j * TB + c < n) ? ld_dev(&X[(long)(j * TB + c) + (long)q * ldx]) : T(0);
+      }
 #pragma unroll
-      for (int c = 0; c < 16; ++c) {
-        const T m = mv[u][c];
-#pragma unroll
-        for (int q = 0; q < TQ; ++q) acc[q] = acc[q] - m * sx[u][16 * g + c][q];
+      for (int it = 0; it < PER; ++it) {
+        const int e = tid + it * 256;
+        if (e < DEPB * TB * QN) sx[e / (TB * QN)][e % TB][(e / TB) % QN] = xs[it];
       }
     }
     __syncthreads();
+    T p[QN];
+#pragma unroll
+    for (int q = 0; q < QN; ++q) p[q] = T(0);
+#pragma unroll
+    for (int u = 0; u < DEPB; ++u) {
+      if (u < nd) {  // uniform: the last dependency's round holds one tile
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+          const T m = mv[u][c];
+#pragma unroll
+          for (int q = 0; q < QN; ++q) p[q] = p[q] - m * sx[u][16 * g + c][q];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QN; ++q) acc[q] = acc[q] + p[q];
+    __syncthreads();
+    d0 += nd;
   }
 
   // partial sums of the four column groups, then y = b + acc
-  T* red = &sx[0][0][0];  // [4][TB][TQ]
+  T* red = &sx[0][0][0];  // [4][TB][QN]
 #pragma unroll
-  for (int q = 0; q < TQ; ++q) red[(g * TB + i) * TQ + q] = acc[q];
+  for (int q = 0; q < QN; ++q) red[(g * TB + i) * QN + q] = acc[q];
   __syncthreads();
-  // wave g solves right-hand sides 4g .. 4g+3 of the diagonal block
+  // wave g solves right-hand sides QW*g .. QW*g+QW-1 of the diagonal block
   const int lane = i;
-  T x[4];
+  T x[QW];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int q = 4 * g + u;
+  for (int u = 0; u < QW; ++u) {
+    const int q = QW * g + u;
     T y = T(0);
-    if (lane < bs && q < nrhs) {
+    if (g < NWQ && lane < bs && q < nrhs) {
       y = X[(long)(r0 + lane) + (long)q * ldx];
-      y = y + (((red[(0 * TB + lane) * TQ + q] + red[(1 * TB + lane) * TQ + q]) + red[(2 * TB + lane) * TQ + q]) +
-               red[(3 * TB + lane) * TQ + q]);
+      y = y + (((red[(0 * TB + lane) * QN + q] + red[(1 * TB + lane) * QN + q]) + red[(2 * TB + lane) * QN + q]) +
+               red[(3 * TB + lane) * QN + q]);
     }
     x[u] = y;
   }
-  if (LOWERM) {
-    for (int k = 0; k < bs; ++k) {
-      const T mik = sD[lane][k];
-      const T dkk = sD[k][k];
+  __syncthreads();  // every partial sum has been read before red is reused
+  if (g < NWQ) {
+    // x = inv(M_dd) y: the wave's y columns through LDS, then row `lane` of
+    // the inverse times them
+    T* sy = red + (size_t)g * TB * QN;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (!UNIT && lane == k && x[u] != T(0)) x[u] = x[u] / dkk;
-        const T xk = __shfl(x[u], k);
-        if (xk != T(0) && lane > k) x[u] = x[u] - xk * mik;
-      }
+    for (int u = 0; u < QW; ++u) sy[lane * QW + u] = x[u];
+    __builtin_amdgcn_wave_barrier();
+    T xs2[QW];
+#pragma unroll
+    for (int u = 0; u < QW; ++u) xs2[u] = T(0);
+#pragma unroll 8
+    for (int k = 0; k < TB; ++k) {
+      const T zik = sD[lane][k];
+#pragma unroll
+      for (int u = 0; u < QW; ++u) xs2[u] = xs2[u] + zik * sy[k * QW + u];
     }
-  } else {
-    for (int k = bs - 1; k >= 0; --k) {
-      const T mik = sD[lane][k];
-      const T dkk = sD[k][k];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (!UNIT && lane == k && x[u] != T(0)) x[u] = x[u] / dkk;
-        const T xk = __shfl(x[u], k);
-        if (xk != T(0) && lane < k) x[u] = x[u] - xk * mik;
-      }
+    for (int u = 0; u < QW; ++u) x[u] = xs2[u];
+#pragma unroll
+    for (int u = 0; u < QW; ++u) {
+      const int q = QW * g + u;
+      if (lane < bs && q < nrhs) st_dev(&X[(long)(r0 + lane) + (long)q * ldx], x[u]);
     }
-  }
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int q = 4 * g + u;
-    if (lane < bs && q < nrhs) st_dev(&X[(long)(r0 + lane) + (long)q * ldx], x[u]);
   }
   wait_vm();
   __syncthreads();
   if (tid == 0) st_dev(&flags[bi], 1u);
+}
+
+template <class T, int QN, bool LOWERM, bool UNIT, bool TRANS>
+void launch_chain_q(int nblk, int n, int nq, const T* A, long lda, T* X, long ldx, unsigned* ctl, int* err,
+                    hipStream_t s) {
+  tri_chain_kernel<T, QN, LOWERM, UNIT, TRANS><<<nblk, 256, 0, s>>>(n, nq, A, lda, X, ldx, ctl, ctl + 1, err);
 }
 
 template <class T, bool LOWERM, bool UNIT, bool TRANS>
@@ -168,8 +258,10 @@ void launch_chain(int n, int nrhs, const T* A, long lda, T* X, long ldx, unsigne
   for (int q0 = 0; q0 < nrhs; q0 += TQ) {
     const int nq = min(TQ, nrhs - q0);
     launch_fill_u32(ctl, (size_t)nblk + 1, 0u, s);  // ticket + flags (a kernel: graph-safe)
-    tri_chain_kernel<T, LOWERM, UNIT, TRANS><<<nblk, 256, 0, s>>>(n, nq, A, lda, X + (long)q0 * ldx, ldx, ctl,
-                                                                  ctl + 1, err);
+    T* Xq = X + (long)q0 * ldx;
+    if (nq == 1) launch_chain_q<T, 1, LOWERM, UNIT, TRANS>(nblk, n, nq, A, lda, Xq, ldx, ctl, err, s);
+    else if (nq <= 4) launch_chain_q<T, 4, LOWERM, UNIT, TRANS>(nblk, n, nq, A, lda, Xq, ldx, ctl, err, s);
+    else launch_chain_q<T, 16, LOWERM, UNIT, TRANS>(nblk, n, nq, A, lda, Xq, ldx, ctl, err, s);
   }
 }
 

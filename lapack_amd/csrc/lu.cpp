@@ -181,8 +181,8 @@ bool is_device_ptr(const void* p) {
 template <class T> struct K;
 template <> struct K<double> {
   static void gemm(char ta, char tb, int m, int n, int k, double al, const double* A, long lda, const double* B,
-                   long ldb, double be, double* C, long ldc, hipStream_t s) {
-    launch_dgemm(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
+                   long ldb, double be, double* C, long ldc, hipStream_t s, bool skip0 = false) {
+    launch_dgemm(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc, s, skip0);
   }
   static void update(int m, int n, int k, const double* A, long lda, const double* B, long ldb, double* C, long ldc,
                      bool skip0, hipStream_t s) {
@@ -192,8 +192,8 @@ template <> struct K<double> {
     launch_transpose_d(r, c, in, li, out, lo, s);
   }
   static void apply_cols(double* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs, const int* np,
-                         hipStream_t s) {
-    launch_apply_pairs_cols_d(T, ldt, r0, r1, r2, r3, pairs, np, s);
+                         hipStream_t s, bool skip0 = false) {
+    launch_apply_pairs_cols_d(T, ldt, r0, r1, r2, r3, pairs, np, s, skip0);
   }
   static void gather(int nr, int nc, const int* perm, const double* src, long ls, double* dst, long ld, hipStream_t s) {
     launch_gather_rows_d(nr, nc, perm, src, ls, dst, ld, s);
@@ -202,12 +202,15 @@ template <> struct K<double> {
                            hipStream_t s) {
     launch_trtri_blocks_d(up, unit, n, nb, U, ldu, W, ldw, s);
   }
-  static void copy(int r, int c, const double* src, long ls, double* dst, long ld, hipStream_t s) {
-    launch_copy_d(r, c, src, ls, dst, ld, s);
+  static void copy(int r, int c, const double* src, long ls, double* dst, long ld, hipStream_t s, bool skip0 = false) {
+    launch_copy_d(r, c, src, ls, dst, ld, s, skip0);
   }
   static int panel(double* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned tagbase, double* x,
                    unsigned* tags, int* err, unsigned* rk, bool xcd0, hipStream_t s) {
     return launch_getf2_panel_d(T, ldt, j0, w, m, ipiv, info, tagbase, x, tags, err, rk, xcd0, s);
+  }
+  static void boundary(double* T, long ldt, int j, int jj, int w, int jend, int m, const int* ipiv, hipStream_t s) {
+    launch_panel_boundary_d(T, ldt, j, jj, w, jend, m, ipiv, s);
   }
 };
 // Operand precision of the FP32 factorization's trailing updates on the
@@ -217,8 +220,8 @@ static thread_local int t_upd_kind = 0;
 
 template <> struct K<float> {
   static void gemm(char ta, char tb, int m, int n, int k, float al, const float* A, long lda, const float* B,
-                   long ldb, float be, float* C, long ldc, hipStream_t s) {
-    launch_sgemm(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
+                   long ldb, float be, float* C, long ldc, hipStream_t s, bool skip0 = false) {
+    launch_sgemm(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc, s, skip0);
   }
   static void update(int m, int n, int k, const float* A, long lda, const float* B, long ldb, float* C, long ldc,
                      bool skip0, hipStream_t s) {
@@ -231,8 +234,8 @@ template <> struct K<float> {
     launch_transpose_s(r, c, in, li, out, lo, s);
   }
   static void apply_cols(float* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs, const int* np,
-                         hipStream_t s) {
-    launch_apply_pairs_cols_s(T, ldt, r0, r1, r2, r3, pairs, np, s);
+                         hipStream_t s, bool skip0 = false) {
+    launch_apply_pairs_cols_s(T, ldt, r0, r1, r2, r3, pairs, np, s, skip0);
   }
   static void gather(int nr, int nc, const int* perm, const float* src, long ls, float* dst, long ld, hipStream_t s) {
     launch_gather_rows_s(nr, nc, perm, src, ls, dst, ld, s);
@@ -241,12 +244,15 @@ template <> struct K<float> {
                            hipStream_t s) {
     launch_trtri_blocks_s(up, unit, n, nb, U, ldu, W, ldw, s);
   }
-  static void copy(int r, int c, const float* src, long ls, float* dst, long ld, hipStream_t s) {
-    launch_copy_s(r, c, src, ls, dst, ld, s);
+  static void copy(int r, int c, const float* src, long ls, float* dst, long ld, hipStream_t s, bool skip0 = false) {
+    launch_copy_s(r, c, src, ls, dst, ld, s, skip0);
   }
   static int panel(float* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned tagbase, double* x,
                    unsigned* tags, int* err, unsigned* rk, bool xcd0, hipStream_t s) {
     return launch_getf2_panel_s(T, ldt, j0, w, m, ipiv, info, tagbase, x, tags, err, rk, xcd0, s);
+  }
+  static void boundary(float* T, long ldt, int j, int jj, int w, int jend, int m, const int* ipiv, hipStream_t s) {
+    launch_panel_boundary_s(T, ldt, j, jj, w, jend, m, ipiv, s);
   }
 };
 
@@ -430,10 +436,6 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
   // DGEMM between leaves (buffers of slot 1, stream st).
   auto factor_panel = [&](int j, int jb, hipStream_t st, PhaseTimer& pt, std::vector<double>& fl,
                           bool xcd0) -> int {
-    T* X = Xs[1];
-    T* W = Ws[1];
-    int* pairs = Ps[1];
-    int* npairs = pairs + 2 * swp_max_pairs();
     for (int jj = j; jj < j + jb;) {
       const int h = m - jj;
       const int w = std::min({tune_leaf(), panel_max_width(h), j + jb - jj});
@@ -444,26 +446,19 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
       endm(pt, fl);
       if (r) return r;
       const int rest = j + jb - jj - w;
+      // interchanges on the panel's other columns and the in-panel DTRSM
+      // (SRC/dgetrf2.f:234-241): one single-workgroup launch
       if (jj > j || rest > 0) {
         mark(pt, fl, LA_PHASE_LASWP, 0);
-        launch_swaps_to_pairs(jj, jj + w, m, ipiv, pairs, npairs, gmap, st);
-        K<T>::apply_cols(Tm, ldt, j, jj, jj + w, j + jb, pairs, npairs, st);
+        K<T>::boundary(Tm, ldt, j, jj, w, j + jb, m, ipiv, st);
         endm(pt, fl);
       }
-      if (rest > 0) {
-        // in-panel DTRSM: T(jj+w:j+jb, jj:jj+w) <- * inv(U'(w x w))
-        mark(pt, fl, LA_PHASE_TRSM, 2.0 * rest * w * w / 2.0);
-        K<T>::trtri_blocks(true, true, w, w, Tp(jj, jj), ldt, W, ldw, st);
-        K<T>::gemm('N', 'N', rest, w, w, T(1), Tp(jj + w, jj), ldt, W, ldw, T(0), X, ldx, st);
-        K<T>::copy(rest, w, X, ldx, Tp(jj + w, jj), ldt, st);
+      // in-panel DGEMM: T(jj+w:j+jb, jj+w:m) -= T(jj+w:j+jb, jj:jj+w) * T(jj:jj+w, jj+w:m)
+      if (rest > 0 && m - jj - w > 0) {
+        mark(pt, fl, LA_PHASE_GEMM_PANEL, 2.0 * rest * (double)(m - jj - w) * w);
+        K<T>::gemm('N', 'N', rest, m - jj - w, w, T(-1), Tp(jj + w, jj), ldt, Tp(jj, jj + w), ldt, T(1),
+                   Tp(jj + w, jj + w), ldt, st);
         endm(pt, fl);
-        // in-panel DGEMM: T(jj+w:j+jb, jj+w:m) -= T(jj+w:j+jb, jj:jj+w) * T(jj:jj+w, jj+w:m)
-        if (m - jj - w > 0) {
-          mark(pt, fl, LA_PHASE_GEMM_PANEL, 2.0 * rest * (double)(m - jj - w) * w);
-          K<T>::gemm('N', 'N', rest, m - jj - w, w, T(-1), Tp(jj + w, jj), ldt, Tp(jj, jj + w), ldt, T(1),
-                     Tp(jj + w, jj + w), ldt, st);
-          endm(pt, fl);
-        }
       }
       jj += w;
     }
@@ -479,30 +474,34 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
     T* W = Ws[0];
     int* pairs = Ps[0];
     int* npairs = pairs + 2 * swp_max_pairs();
-    // ---- interchanges outside the panel (SRC/dgetrf.f:193,199-200)
-    mark(pa, fla, LA_PHASE_LASWP, 0);
-    if (j > 0 || j + jb < n) {
-      launch_swaps_to_pairs(j, j + jb, m, ipiv, pairs, npairs, gmap, s);
-      K<T>::apply_cols(Tm, ldt, 0, j, j + jb, n, pairs, npairs, s);
-    }
-    endm(pa, fla);
-    if (j + jb >= n) continue;
-    const int n2 = n - j - jb;
-    // ---- DTRSM: U12^T = A12^T * inv(L11^T)  (SRC/dgetrf.f:204-207)
-    mark(pa, fla, LA_PHASE_TRSM, 2.0 * n2 * jb * (double)jb / 2.0);
-    K<T>::trtri_blocks(true, true, jb, jb, Tp(j, j), ldt, W, ldw, s);
-    K<T>::gemm('N', 'N', n2, jb, jb, T(1), Tp(j + jb, j), ldt, W, ldw, T(0), X, ldx, s);
-    K<T>::copy(n2, jb, X, ldx, Tp(j + jb, j), ldt, s);
-    endm(pa, fla);
-    if (j + jb >= m) continue;
-    // ---- DGEMM: A22 -= A21*U12, i.e. T22 -= T12 * T21 (SRC/dgetrf.f:212-216),
-    //      the next panel's columns first
+    // Only the next panel's columns are on the critical path: their
+    // interchanges, TRSM and update run first and release the panel stream;
+    // the interchanges left of the panel and the TRSM / update of the
+    // columns right of the next panel follow on s, beside that panel.
     const int jn = j + jb;
+    const int n2 = n - jn;
     const int jbn = jn < mn ? std::min(nb, mn - jn) : 0;
     const int mrows = m - jn;
     // the next panel takes XCD 0 for itself when it runs multi-workgroup
     const bool split = xcdsplit && jbn > 0 && panel_groups(m - jn) > 0 && panel_groups(m - jn) <= 32;
-    if (jbn > 0) {
+    // ---- interchanges outside the panel (SRC/dgetrf.f:193,199-200): next panel's columns
+    mark(pa, fla, LA_PHASE_LASWP, 0);
+    if (j > 0 || jn < n) launch_swaps_to_pairs(j, jn, m, ipiv, pairs, npairs, gmap, s);
+    if (jbn > 0) K<T>::apply_cols(Tm, ldt, jn, jn + jbn, 0, 0, pairs, npairs, s);
+    endm(pa, fla);
+    if (n2 > 0) {
+      // ---- DTRSM: U12^T = A12^T * inv(L11^T)  (SRC/dgetrf.f:204-207), next panel's columns
+      mark(pa, fla, LA_PHASE_TRSM, 2.0 * jbn * jb * (double)jb / 2.0);
+      K<T>::trtri_blocks(true, true, jb, jb, Tp(j, j), ldt, W, ldw, s);
+      if (jbn > 0) {
+        K<T>::gemm('N', 'N', jbn, jb, jb, T(1), Tp(jn, j), ldt, W, ldw, T(0), X, ldx, s);
+        K<T>::copy(jbn, jb, X, ldx, Tp(jn, j), ldt, s);
+      }
+      endm(pa, fla);
+    }
+    // ---- DGEMM: A22 -= A21*U12, i.e. T22 -= T12 * T21 (SRC/dgetrf.f:212-216),
+    //      the next panel's columns, then that panel on the panel stream
+    if (jbn > 0 && mrows > 0) {
       mark(pa, fla, LA_PHASE_GEMM, 2.0 * jbn * (double)mrows * jb);
       K<T>::update(jbn, mrows, jb, X, ldx, Tp(j, jn), ldt, Tp(jn, jn), ldt, false, s);
       endm(pa, fla);
@@ -512,10 +511,22 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
       if (rc) break;
       LA_TRY(hipEventRecord(c->ev[0], sp));
     }
-    if (n2 - jbn > 0) {
-      mark(pa, fla, LA_PHASE_GEMM, 2.0 * (n2 - jbn) * (double)mrows * jb);
-      K<T>::update(n2 - jbn, mrows, jb, X + jbn, ldx, Tp(j, jn), ldt, Tp(jn + jbn, jn), ldt, split, s);
+    // ---- the rest, beside the next panel: interchanges left of this panel
+    //      and right of the next one, their TRSM and update
+    mark(pa, fla, LA_PHASE_LASWP, 0);
+    if (j > 0 || jn + jbn < n) K<T>::apply_cols(Tm, ldt, 0, j, jn + jbn, n, pairs, npairs, s, split);
+    endm(pa, fla);
+    const int nr = n2 - jbn;
+    if (nr > 0) {
+      mark(pa, fla, LA_PHASE_TRSM, 2.0 * nr * jb * (double)jb / 2.0);
+      K<T>::gemm('N', 'N', nr, jb, jb, T(1), Tp(jn + jbn, j), ldt, W, ldw, T(0), X + jbn, ldx, s, split);
+      K<T>::copy(nr, jb, X + jbn, ldx, Tp(jn + jbn, j), ldt, s, split);
       endm(pa, fla);
+      if (mrows > 0) {
+        mark(pa, fla, LA_PHASE_GEMM, 2.0 * nr * (double)mrows * jb);
+        K<T>::update(nr, mrows, jb, X + jbn, ldx, Tp(j, jn), ldt, Tp(jn + jbn, jn), ldt, split, s);
+        endm(pa, fla);
+      }
     }
   }
   if (rc) {

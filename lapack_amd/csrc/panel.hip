@@ -995,6 +995,7 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
   }
 }
 
+
 }  // namespace
 
 // Launch configurations (measured on MI355X, per column of a 64-wide leaf):
@@ -1017,8 +1018,7 @@ size_t panel_tag_count() { return GMAX; }
 int panel_groups(int h) { return h <= 1024 ? 0 : (h + 511) / 512; }
 
 // Multi-workgroup panel variant: getf2_panel_w8_kernel by default;
-// LAPACK_AMD_PANEL=ll8 / ll / tag select the per-wave-slot unrolled kernel,
-// the per-wave-slot rotating kernel, or the tag-word exchange of
+// LAPACK_AMD_PANEL=ll8 / ll / tag select the per-wave-slot unrolled kernel, the per-wave-slot rotating kernel, or the tag-word exchange of
 // getf2_panel_kernel (kept for comparison).
 static int panel_variant() {
   static int v = -1;

@@ -64,11 +64,16 @@ def test_geequ_zero_row_and_column(dev):
     assert info == 40 + 12 == oracle.dgeequ(a, True)[5]
 
 
-def conditioned(n, cond, seed):
-    """A = Q1 diag(s) Q2^T, Q from Householder QR of seeded Gaussians, s geometric 1 .. 1/cond."""
+def conditioned(n, cond, seed, mode=3):
+    """A = Q1 diag(s) Q2^T, Q from Householder QR of seeded Gaussians; the
+    singular values follow DLATMS's modes (TESTING/MATGEN/dlatms.f:158-161):
+    3 geometric 1 .. 1/cond, 4 arithmetic 1 .. 1/cond."""
     q1, _ = np.linalg.qr(la.matrix(3, seed, n, n))
     q2, _ = np.linalg.qr(la.matrix(3, seed + 1, n, n))
-    s = np.logspace(0, -np.log10(cond), n)
+    if mode == 3:
+        s = np.logspace(0, -np.log10(cond), n)
+    else:
+        s = 1.0 - np.arange(n) / (n - 1.0) * (1.0 - 1.0 / cond)
     return np.asfortranarray((q1 * s) @ q2.T)
 
 
@@ -110,10 +115,10 @@ def test_ir_well_conditioned(dev, prec):
 @pytest.mark.parametrize("prec", ["S", "H", "B"])
 def test_ir_conditioned_1e6(dev, prec):
     """The conditioned variant of the mixed-precision workload (kappa = 1e6) at
-    n = 1024: FP16 and BF16 factors cannot refine classically (kappa * u > 1),
-    GMRES-IR must carry them."""
+    n = 1024, arithmetic spectrum: FP16 and BF16 factors cannot refine
+    classically (kappa * u > 1), GMRES-IR must carry them."""
     n = 1024
-    a = conditioned(n, 1e6, 710)
+    a = conditioned(n, 1e6, 710, mode=4)
     b = la.matrix(3, 712, n, 1)
     x, it, git, info, berr = solve_ir(a, b, prec)
     assert info == 0
@@ -121,6 +126,32 @@ def test_ir_conditioned_1e6(dev, prec):
     if prec in ("H", "B"):
         assert git > 0
     print(f"prec {prec}: iter {it}, gmres {git}, berr {berr.max():.2e}, fwd {fwd:.2e}")
+
+
+@pytest.mark.parametrize("prec", ["S", "H", "B"])
+def test_ir_conditioned_geometric_contract(dev, prec):
+    """kappa = 1e6 with a geometric spectrum: about half the singular values
+    lie below the FP16/BF16 unit roundoff, so the preconditioned operator has
+    hundreds of outlying eigenvalues and GMRES(50) may not reach the stopping
+    test within its 30 cycles.  The contract is DSGESV's
+    (SRC/dsgesv.f:413-427): either convergence with ITER >= 0, or ITER = -31
+    and the FP64 LU's solution."""
+    n = 1024
+    a = conditioned(n, 1e6, 710, mode=3)
+    b = la.matrix(3, 712, n, 1)
+    x, it, git, info, berr = solve_ir(a, b, prec)
+    assert info == 0
+    if prec == "S":
+        check_solution(a, b, x, berr, it)
+    elif it >= 0:
+        check_solution(a, b, x, berr, it)
+    else:
+        assert it == -31 and git > 0
+        lu, p, _ = oracle.dgetrf(a)
+        xr, _ = oracle.dgetrs(lu, p, b)
+        assert oracle.dget02(a, x, b) < 30.0
+        assert np.max(np.abs(x - xr)) <= np.linalg.cond(a, np.inf) * n * EPS * np.max(np.abs(xr))
+    print(f"prec {prec}: iter {it}, gmres {git}")
 
 
 def test_ir_fallbacks_and_errors(dev):
