@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--mp-n", type=int, default=32768, help="order of the mixed-precision comparison")
     p.add_argument("--no-mp", action="store_true", help="skip the mixed-precision comparison")
+    p.add_argument("--no-phase", action="store_true",
+                   help="time the steps without per-phase events (no roofline / phase fields)")
     p.add_argument("--mp-half", action="store_true",
                    help="add the FP16 / BF16 GMRES-IR variants to the mixed-precision comparison")
     p.add_argument("--calib-copy-mib", type=int, default=0,
@@ -70,6 +72,20 @@ def cpu_baseline(n_sample, seed):
         "kind": kind,
         "sample": f"DGETRF n={n_sample} N(0,1), reference BLAS, best of 2, {best:.2f} s",
     }
+
+
+def update_traffic(n):
+    """HBM bytes per trailing-update launch measured by the two rocprofv3 PMC
+    passes (FETCH_SIZE, WRITE_SIZE, copy-calibrated) summarised in profiles/,
+    for the configuration they were taken on; None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"dgetrf_n{n}_update_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def mp_compare(n, seed, half=False):
@@ -194,8 +210,11 @@ def main():
     torch.cuda.synchronize()
     print("[bench] warmup done", file=sys.stderr, flush=True)
 
+    # timed steps: events only around the trailing-update launches (the
+    # roofline's kernel); the full phase breakdown comes from one more,
+    # untimed step afterwards
     la.phase_reset()
-    la.phase_enable(True)
+    la.phase_enable(not args.no_phase, level=2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(K):
@@ -203,7 +222,17 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     la.phase_enable(False)
-    ph = la.phase_read()
+    ph_upd = la.phase_read()
+    ph = ph_upd
+    if not args.no_phase:
+        d_as[0].copy_(d_a0)
+        d_bs[0].copy_(d_b0)
+        la.phase_reset()
+        la.phase_enable(True, level=1)
+        step(0)
+        torch.cuda.synchronize()
+        la.phase_enable(False)
+        ph = la.phase_read()
     infos = d_info.cpu().numpy()
     assert (infos == 0).all(), infos
 
@@ -220,7 +249,7 @@ def main():
     anorm = a0.abs().sum(dim=1).max().item()  # ||A||_1 = max column sum
     ratio = (r.abs().sum(dim=1) / (anorm * x.abs().sum(dim=1) * 2.0 ** -53)).max().item()
 
-    gemm_ms, gemm_calls, gemm_flops = ph["gemm"]
+    gemm_ms, gemm_calls, gemm_flops = ph_upd["gemm"]
     # DGETRF and DGETRS times of the timed steps (events on the caller's stream)
     getrf_ms = sum(evs[i][0].elapsed_time(evs[i][1]) for i in range(K))
     getrs_ms = sum(evs[i][1].elapsed_time(evs[i][2]) for i in range(K))
@@ -250,7 +279,7 @@ def main():
             "peak": PEAK_FP64,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64, 4),
-            "traffic": None,
+            "traffic": update_traffic(n) if nrhs == 16 else None,
             "launches": gemm_calls,
             "avg_launch_ms": round(gemm_ms / max(gemm_calls, 1), 4),
         },
@@ -258,7 +287,7 @@ def main():
         "getrf_frac_of_fp64_peak": round(K * f_getrf / (getrf_ms * 1e-3) / 1e12 / PEAK_FP64, 4) if getrf_ms > 0 else None,
         "getrf_ms_per_step": round(getrf_ms / K, 3),
         "getrs_ms_per_step": round(getrs_ms / K, 3),
-        "phase_ms_per_step": {p: round(ph[p][0] / K, 3) for p in ph},
+        "phase_ms_one_step": {p: round(ph[p][0], 3) for p in ph},
         "dget02_ratio": round(ratio, 4),
     }
     if not args.no_mp:

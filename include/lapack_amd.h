@@ -222,7 +222,9 @@ int lapack_amd_xerbla_last(char* name_out, int cap);
 int lapack_amd_xerbla_calls(void);
 
 /* Per-phase GPU time of getrf/getrs, accumulated over calls while enabled
- * (hipEvents on the call's stream). */
+ * (hipEvents on the stream each phase runs on).  lapack_amd_phase_enable:
+ * 0 off, 1 every phase, 2 only the trailing update (LA_PHASE_GEMM: two
+ * events per update launch, little cost for a timed run). */
 enum {
   LA_PHASE_TRANSPOSE = 0,
   LA_PHASE_PANEL = 1,

@@ -315,8 +315,10 @@ def xerbla_last():
     return buf.value.decode(), info, lib().lapack_amd_xerbla_calls()
 
 
-def phase_enable(on=True):
-    lib().lapack_amd_phase_enable(1 if on else 0)
+def phase_enable(on=True, level=1):
+    """level 1: every phase; level 2: only the trailing update (cheap enough
+    to stay on in a timed run)."""
+    lib().lapack_amd_phase_enable(int(level) if on else 0)
 
 
 def phase_reset():

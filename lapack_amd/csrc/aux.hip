@@ -190,13 +190,13 @@ template <class T>
 __global__ __launch_bounds__(256) void apply_pairs_cols_kernel(T* __restrict__ Tm, long ldt, int r0, int r1, int r2,
                                                                int r3, const int* __restrict__ pairs,
                                                                const int* __restrict__ npairs, int nblk,
-                                                               int skip0) {
+                                                               int nskip) {
   __shared__ T st[SWP_MAXPAIRS * SWP_ROWS];
-  // skip0: blocks dispatched to XCD 0 (blockIdx % 8 == 0) leave at once
+  // nskip: blocks dispatched to XCDs 0 .. nskip-1 (blockIdx % 8) leave at once
   int blk = blockIdx.x;
-  if (skip0) {
-    if (blk % 8 == 0) return;
-    blk = (blk / 8) * 7 + (blk % 8 - 1);
+  if (nskip) {
+    if (blk % 8 < nskip) return;
+    blk = (blk / 8) * (8 - nskip) + (blk % 8 - nskip);
   }
   if (blk >= nblk) return;
   const int np = *npairs;
@@ -501,13 +501,13 @@ __global__ __launch_bounds__(256) void trtri_kernel(int n, int nb, const T* __re
 // ---- copies and fills -----------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void copy_kernel(int rows, int cols, const T* __restrict__ src, long lds,
-                                                   T* __restrict__ dst, long ldd, int skip0) {
-  // skip0: blocks dispatched to XCD 0 (blockIdx % 8 == 0) leave at once
+                                                   T* __restrict__ dst, long ldd, int nskip) {
+  // nskip: blocks dispatched to XCDs 0 .. nskip-1 (blockIdx % 8) leave at once
   long lb = blockIdx.x, ng = gridDim.x;
-  if (skip0) {
-    if (blockIdx.x % 8 == 0) return;
-    lb = (long)(blockIdx.x / 8) * 7 + (blockIdx.x % 8 - 1);
-    ng = (long)(gridDim.x / 8) * 7;
+  if (nskip) {
+    if ((int)(blockIdx.x % 8) < nskip) return;
+    lb = (long)(blockIdx.x / 8) * (8 - nskip) + (blockIdx.x % 8 - nskip);
+    ng = (long)(gridDim.x / 8) * (8 - nskip);
   }
   const long tot = (long)rows * cols;
   for (long e = lb * 256 + threadIdx.x; e < tot; e += ng * 256) {
@@ -572,21 +572,21 @@ void launch_swaps_to_pairs(int k1, int k2, int npos, const int* ipiv, int* pairs
 }
 template <class T>
 static void apply_cols_t(T* Tm, long ldt, int r0, int r1, int r2, int r3, const int* pairs, const int* npairs,
-                         hipStream_t s, bool skip0) {
+                         hipStream_t s, int nskip) {
   if (r1 < r0) r1 = r0;
   if (r3 < r2) r3 = r2;
   const int nb = (r1 - r0 + SWP_ROWS - 1) / SWP_ROWS + (r3 - r2 + SWP_ROWS - 1) / SWP_ROWS;
-  const int grid = skip0 ? (nb + 6) / 7 * 8 : nb;
+  const int grid = nskip ? (nb + (8 - nskip) - 1) / (8 - nskip) * 8 : nb;
   if (nb > 0)
-    apply_pairs_cols_kernel<T><<<grid, 256, 0, s>>>(Tm, ldt, r0, r1, r2, r3, pairs, npairs, nb, skip0 ? 1 : 0);
+    apply_pairs_cols_kernel<T><<<grid, 256, 0, s>>>(Tm, ldt, r0, r1, r2, r3, pairs, npairs, nb, nskip);
 }
 void launch_apply_pairs_cols_d(double* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs,
-                               const int* npairs, hipStream_t s, bool skip0) {
-  apply_cols_t<double>(T, ldt, r0, r1, r2, r3, pairs, npairs, s, skip0);
+                               const int* npairs, hipStream_t s, int nskip) {
+  apply_cols_t<double>(T, ldt, r0, r1, r2, r3, pairs, npairs, s, nskip);
 }
 void launch_apply_pairs_cols_s(float* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs,
-                               const int* npairs, hipStream_t s, bool skip0) {
-  apply_cols_t<float>(T, ldt, r0, r1, r2, r3, pairs, npairs, s, skip0);
+                               const int* npairs, hipStream_t s, int nskip) {
+  apply_cols_t<float>(T, ldt, r0, r1, r2, r3, pairs, npairs, s, nskip);
 }
 void launch_panel_boundary_d(double* T, long ldt, int j, int jj, int w, int jend, int m, const int* ipiv,
                              hipStream_t s) {
@@ -660,19 +660,19 @@ void launch_trtri_un_s(int nb, const float* U, long ldu, float* W, long ldw, hip
   launch_trtri_t<float>(true, false, nb, nb, U, ldu, W, ldw, s);
 }
 
-static int copy_grid(long work, bool skip0) {
+static int copy_grid(long work, int nskip) {
   const int g = grid_for(work);
-  return skip0 ? (g + 6) / 7 * 8 : g;
+  return nskip ? (g + (8 - nskip) - 1) / (8 - nskip) * 8 : g;
 }
 void launch_copy_d(int rows, int cols, const double* src, long lds, double* dst, long ldd, hipStream_t s,
-                   bool skip0) {
+                   int nskip) {
   if (rows > 0 && cols > 0)
-    copy_kernel<double><<<copy_grid((long)rows * cols, skip0), 256, 0, s>>>(rows, cols, src, lds, dst, ldd, skip0);
+    copy_kernel<double><<<copy_grid((long)rows * cols, nskip), 256, 0, s>>>(rows, cols, src, lds, dst, ldd, nskip);
 }
 void launch_copy_s(int rows, int cols, const float* src, long lds, float* dst, long ldd, hipStream_t s,
-                   bool skip0) {
+                   int nskip) {
   if (rows > 0 && cols > 0)
-    copy_kernel<float><<<copy_grid((long)rows * cols, skip0), 256, 0, s>>>(rows, cols, src, lds, dst, ldd, skip0);
+    copy_kernel<float><<<copy_grid((long)rows * cols, nskip), 256, 0, s>>>(rows, cols, src, lds, dst, ldd, nskip);
 }
 void launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s) {
   if (n) fill_kernel<unsigned><<<grid_for((long)n), 256, 0, s>>>(p, n, v);

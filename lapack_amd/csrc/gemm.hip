@@ -329,13 +329,13 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
 
 template <class T>
 void launch_gemm_t(char ta, char tb, int m, int n, int k, T alpha, const T* A, long lda, const T* B,
-                   long ldb, T beta, T* C, long ldc, hipStream_t s, bool skip0) {
+                   long ldb, T beta, T* C, long ldc, hipStream_t s, int nskip) {
   if (m <= 0 || n <= 0) return;
   // BLAS quick return (BLAS/SRC/dgemm.f:298-299).
   if ((alpha == T(0) || k <= 0) && beta == T(1)) return;
   const int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
   const int tiles = tm * tn;
-  const int nx = skip0 ? NXCD - 1 : NXCD;
+  const int nx = NXCD - nskip;
   const int chunk = (tiles + nx - 1) / nx;
   const dim3 grid(chunk * NXCD);
   const bool TA = (ta == 'T' || ta == 't' || ta == 'C' || ta == 'c');
@@ -364,12 +364,12 @@ static int update_cus_per_xcd() {
 }
 
 template <class T>
-void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long ldb, T* C, long ldc, bool skip0,
+void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long ldb, T* C, long ldc, int nskip,
                      hipStream_t s) {
   if (m <= 0 || n <= 0 || k <= 0) return;
-  const int nx = skip0 ? NXCD - 1 : NXCD;
+  const int nx = NXCD - nskip;
   // 256x128 tiles once they still give every CU two workgroups or more.
-  // Beside a panel (skip0) an update that is not much longer than the
+  // Beside a panel (nskip > 0) an update that is not much longer than the
   // panel chain runs persistent on at most update_cus_per_xcd() CUs of each
   // XCD, so the panel's own short launches find CUs at once instead of
   // queueing behind the update's workgroups.
@@ -377,7 +377,7 @@ void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long
   if ((long)tmb * tnb >= 512) {
     const int chunk = (tmb * tnb + nx - 1) / nx;
     int per = chunk;
-    if (skip0 && 2.0 * m * (double)n * k < 6e10) per = std::min(chunk, update_cus_per_xcd());
+    if (nskip > 0 && 2.0 * m * (double)n * k < 6e10) per = std::min(chunk, update_cus_per_xcd());
     update_big_kernel<T, 4, 2><<<dim3(per * NXCD), 512, 0, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tmb, tnb, chunk,
                                                                nx);
     return;
@@ -391,20 +391,20 @@ void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long
 }  // namespace
 
 void launch_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double* A, long lda,
-                  const double* B, long ldb, double beta, double* C, long ldc, hipStream_t s, bool skip0) {
-  launch_gemm_t<double>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, skip0);
+                  const double* B, long ldb, double beta, double* C, long ldc, hipStream_t s, int nskip) {
+  launch_gemm_t<double>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, nskip);
 }
 void launch_dgemm_update(int m, int n, int k, const double* A, long lda, const double* B, long ldb, double* C,
-                         long ldc, bool skip0, hipStream_t s) {
-  launch_update_t<double>(m, n, k, A, lda, B, ldb, C, ldc, skip0, s);
+                         long ldc, int nskip, hipStream_t s) {
+  launch_update_t<double>(m, n, k, A, lda, B, ldb, C, ldc, nskip, s);
 }
 void launch_sgemm_update(int m, int n, int k, const float* A, long lda, const float* B, long ldb, float* C, long ldc,
-                         bool skip0, hipStream_t s) {
-  launch_update_t<float>(m, n, k, A, lda, B, ldb, C, ldc, skip0, s);
+                         int nskip, hipStream_t s) {
+  launch_update_t<float>(m, n, k, A, lda, B, ldb, C, ldc, nskip, s);
 }
 void launch_sgemm(char ta, char tb, int m, int n, int k, float alpha, const float* A, long lda,
-                  const float* B, long ldb, float beta, float* C, long ldc, hipStream_t s, bool skip0) {
-  launch_gemm_t<float>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, skip0);
+                  const float* B, long ldb, float beta, float* C, long ldc, hipStream_t s, int nskip) {
+  launch_gemm_t<float>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, nskip);
 }
 
 }  // namespace la

@@ -180,10 +180,10 @@ __global__ __launch_bounds__(HNT, 2) void gemm_update_h_kernel(int M, int N, int
 
 // kind: 1 = FP16 operands, 2 = BF16 operands (FP32 accumulation either way).
 void launch_sgemm_update_half(int kind, int m, int n, int k, const float* A, long lda, const float* B, long ldb,
-                              float* C, long ldc, bool skip0, hipStream_t s) {
+                              float* C, long ldc, int nskip, hipStream_t s) {
   if (m <= 0 || n <= 0 || k <= 0) return;
   const int tm = (m + HBM_ - 1) / HBM_, tn = (n + HBN - 1) / HBN;
-  const int nx = skip0 ? HNXCD - 1 : HNXCD;
+  const int nx = HNXCD - nskip;
   const int chunk = (tm * tn + nx - 1) / nx;
   if (kind == 2)
     gemm_update_h_kernel<2><<<dim3(chunk * HNXCD), HNT, 0, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tm, tn, chunk, nx);

@@ -36,20 +36,20 @@ __device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); 
 
 // C <- alpha*op(A)*op(B) + beta*C, column-major, FP64 MFMA.
 void launch_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double* A, long lda,
-                  const double* B, long ldb, double beta, double* C, long ldc, hipStream_t s, bool skip0 = false);
-// The DGETRF trailing update C <- C - A*B (its own kernel symbol).
-// skip_xcd0: leave XCD 0 to a concurrent panel kernel.
+                  const double* B, long ldb, double beta, double* C, long ldc, hipStream_t s, int nskip = 0);
+// The DGETRF trailing update C <- C - A*B (its own kernel symbols).
+// nskip: leave XCDs 0 .. nskip-1 to a concurrent panel kernel.
 void launch_dgemm_update(int m, int n, int k, const double* A, long lda, const double* B, long ldb, double* C,
-                         long ldc, bool skip_xcd0, hipStream_t s);
+                         long ldc, int nskip, hipStream_t s);
 // FP32 update with operands rounded to FP16 (kind 1) or BF16 (kind 2), FP32
 // accumulation on the half-precision matrix cores (gemm_h.hip).
 void launch_sgemm_update_half(int kind, int m, int n, int k, const float* A, long lda, const float* B, long ldb,
-                              float* C, long ldc, bool skip0, hipStream_t s);
+                              float* C, long ldc, int nskip, hipStream_t s);
 void launch_sgemm_update(int m, int n, int k, const float* A, long lda, const float* B, long ldb, float* C, long ldc,
-                         bool skip_xcd0, hipStream_t s);
+                         int nskip, hipStream_t s);
 // Same in FP32 (MFMA f32).
 void launch_sgemm(char ta, char tb, int m, int n, int k, float alpha, const float* A, long lda,
-                  const float* B, long ldb, float beta, float* C, long ldc, hipStream_t s, bool skip0 = false);
+                  const float* B, long ldb, float beta, float* C, long ldc, hipStream_t s, int nskip = 0);
 
 // out(j,i) = in(i,j): in is rows x cols with ld_in, out cols x rows with ld_out.
 void launch_transpose_d(int rows, int cols, const double* in, long ld_in, double* out, long ld_out,
@@ -68,9 +68,9 @@ int swp_max_pairs();
 void launch_swaps_to_pairs(int k1, int k2, int npos, const int* ipiv, int* pairs, int* npairs, int* gmap,
                            hipStream_t s);
 void launch_apply_pairs_cols_d(double* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs,
-                               const int* npairs, hipStream_t s, bool skip0 = false);
+                               const int* npairs, hipStream_t s, int nskip = 0);
 void launch_apply_pairs_cols_s(float* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs,
-                               const int* npairs, hipStream_t s, bool skip0 = false);
+                               const int* npairs, hipStream_t s, int nskip = 0);
 // Between two leaves of a panel: leaf [jj, jj+w)'s interchanges applied to
 // T rows [j, jj) and [jj+w, jend), then the in-panel DTRSM of T rows
 // [jj+w, jend); one workgroup.  w <= 64.
@@ -103,12 +103,12 @@ void launch_trtri_un_d(int nb, const double* U, long ldu, double* W, long ldw, h
 void launch_trtri_un_s(int nb, const float* U, long ldu, float* W, long ldw, hipStream_t s);
 
 // Copy a rows x cols column-major block.
-// skip0 (copy, interchanges, GEMM): no work on XCD 0, which a concurrent
-// panel keeps for itself.
+// nskip (copy, interchanges, GEMM, update): no work on XCDs 0 .. nskip-1,
+// which a concurrent panel keeps for itself.
 void launch_copy_d(int rows, int cols, const double* src, long lds, double* dst, long ldd, hipStream_t s,
-                   bool skip0 = false);
+                   int nskip = 0);
 void launch_copy_s(int rows, int cols, const float* src, long lds, float* dst, long ldd, hipStream_t s,
-                   bool skip0 = false);
+                   int nskip = 0);
 void launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s);
 void launch_fill_i32(int* p, size_t n, int v, hipStream_t s);
 // info <- LA_ERR_SYNC when a panel barrier timed out (err != 0).

@@ -181,19 +181,19 @@ bool is_device_ptr(const void* p) {
 template <class T> struct K;
 template <> struct K<double> {
   static void gemm(char ta, char tb, int m, int n, int k, double al, const double* A, long lda, const double* B,
-                   long ldb, double be, double* C, long ldc, hipStream_t s, bool skip0 = false) {
-    launch_dgemm(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc, s, skip0);
+                   long ldb, double be, double* C, long ldc, hipStream_t s, int nskip = 0) {
+    launch_dgemm(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc, s, nskip);
   }
   static void update(int m, int n, int k, const double* A, long lda, const double* B, long ldb, double* C, long ldc,
-                     bool skip0, hipStream_t s) {
-    launch_dgemm_update(m, n, k, A, lda, B, ldb, C, ldc, skip0, s);
+                     int nskip, hipStream_t s) {
+    launch_dgemm_update(m, n, k, A, lda, B, ldb, C, ldc, nskip, s);
   }
   static void transpose(int r, int c, const double* in, long li, double* out, long lo, hipStream_t s) {
     launch_transpose_d(r, c, in, li, out, lo, s);
   }
   static void apply_cols(double* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs, const int* np,
-                         hipStream_t s, bool skip0 = false) {
-    launch_apply_pairs_cols_d(T, ldt, r0, r1, r2, r3, pairs, np, s, skip0);
+                         hipStream_t s, int nskip = 0) {
+    launch_apply_pairs_cols_d(T, ldt, r0, r1, r2, r3, pairs, np, s, nskip);
   }
   static void gather(int nr, int nc, const int* perm, const double* src, long ls, double* dst, long ld, hipStream_t s) {
     launch_gather_rows_d(nr, nc, perm, src, ls, dst, ld, s);
@@ -202,8 +202,8 @@ template <> struct K<double> {
                            hipStream_t s) {
     launch_trtri_blocks_d(up, unit, n, nb, U, ldu, W, ldw, s);
   }
-  static void copy(int r, int c, const double* src, long ls, double* dst, long ld, hipStream_t s, bool skip0 = false) {
-    launch_copy_d(r, c, src, ls, dst, ld, s, skip0);
+  static void copy(int r, int c, const double* src, long ls, double* dst, long ld, hipStream_t s, int nskip = 0) {
+    launch_copy_d(r, c, src, ls, dst, ld, s, nskip);
   }
   static int panel(double* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned tagbase, double* x,
                    unsigned* tags, int* err, unsigned* rk, bool xcd0, hipStream_t s) {
@@ -220,22 +220,22 @@ static thread_local int t_upd_kind = 0;
 
 template <> struct K<float> {
   static void gemm(char ta, char tb, int m, int n, int k, float al, const float* A, long lda, const float* B,
-                   long ldb, float be, float* C, long ldc, hipStream_t s, bool skip0 = false) {
-    launch_sgemm(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc, s, skip0);
+                   long ldb, float be, float* C, long ldc, hipStream_t s, int nskip = 0) {
+    launch_sgemm(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc, s, nskip);
   }
   static void update(int m, int n, int k, const float* A, long lda, const float* B, long ldb, float* C, long ldc,
-                     bool skip0, hipStream_t s) {
+                     int nskip, hipStream_t s) {
     if (t_upd_kind == 1 || t_upd_kind == 2)
-      launch_sgemm_update_half(t_upd_kind, m, n, k, A, lda, B, ldb, C, ldc, skip0, s);
+      launch_sgemm_update_half(t_upd_kind, m, n, k, A, lda, B, ldb, C, ldc, nskip, s);
     else
-      launch_sgemm_update(m, n, k, A, lda, B, ldb, C, ldc, skip0, s);
+      launch_sgemm_update(m, n, k, A, lda, B, ldb, C, ldc, nskip, s);
   }
   static void transpose(int r, int c, const float* in, long li, float* out, long lo, hipStream_t s) {
     launch_transpose_s(r, c, in, li, out, lo, s);
   }
   static void apply_cols(float* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs, const int* np,
-                         hipStream_t s, bool skip0 = false) {
-    launch_apply_pairs_cols_s(T, ldt, r0, r1, r2, r3, pairs, np, s, skip0);
+                         hipStream_t s, int nskip = 0) {
+    launch_apply_pairs_cols_s(T, ldt, r0, r1, r2, r3, pairs, np, s, nskip);
   }
   static void gather(int nr, int nc, const int* perm, const float* src, long ls, float* dst, long ld, hipStream_t s) {
     launch_gather_rows_s(nr, nc, perm, src, ls, dst, ld, s);
@@ -244,8 +244,8 @@ template <> struct K<float> {
                            hipStream_t s) {
     launch_trtri_blocks_s(up, unit, n, nb, U, ldu, W, ldw, s);
   }
-  static void copy(int r, int c, const float* src, long ls, float* dst, long ld, hipStream_t s, bool skip0 = false) {
-    launch_copy_s(r, c, src, ls, dst, ld, s, skip0);
+  static void copy(int r, int c, const float* src, long ls, float* dst, long ld, hipStream_t s, int nskip = 0) {
+    launch_copy_s(r, c, src, ls, dst, ld, s, nskip);
   }
   static int panel(float* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned tagbase, double* x,
                    unsigned* tags, int* err, unsigned* rk, bool xcd0, hipStream_t s) {
@@ -259,18 +259,31 @@ template <> struct K<float> {
 // --------------------------------------------------------------- GETRF --
 
 // Phase timing (bench/profiling): events around the phases of getrf.
+// Level 1 times every phase; level 2 only the trailing update (two events
+// per update launch), which keeps the events' own cost out of a timed run.
 struct PhaseTimer {
   bool on = false;
+  int level = 1;
+  bool skipping = false;  // the open interval is not recorded
   hipStream_t s = nullptr;
   std::vector<hipEvent_t> ev;
   std::vector<int> tag;
-  void mark(int t) {
-    if (!on) return;
+  // t >= 0 opens an interval of phase t, t < 0 closes it; true if recorded
+  bool mark(int t) {
+    if (!on) return false;
+    if (t >= 0) {
+      skipping = level == 2 && t != LA_PHASE_GEMM;
+      if (skipping) return false;
+    } else if (skipping) {
+      skipping = false;
+      return false;
+    }
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
+    if (hipEventCreate(&e) != hipSuccess) { on = false; return false; }
     hipEventRecord(e, s);
     ev.push_back(e);
     tag.push_back(t);
+    return true;
   }
 };
 static std::atomic<int> g_phase_on{0};
@@ -407,15 +420,14 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
 
   PhaseTimer pa, pp;  // main stream, panel stream
   pa.on = pp.on = g_phase_on.load() != 0;
+  pa.level = pp.level = g_phase_on.load();
   pa.s = s;
   pp.s = sp;
   std::vector<double> fla, flp;
   // Phase intervals: begin(tag) ... end() around the launches of one phase,
   // so time spent waiting on the other stream is not attributed to it.
   auto mark = [&](PhaseTimer& pt, std::vector<double>& fl, int tag, double flops) {
-    if (!pt.on) return;
-    pt.mark(tag);
-    fl.push_back(flops);
+    if (pt.mark(tag)) fl.push_back(flops);
   };
   auto endm = [&](PhaseTimer& pt, std::vector<double>& fl) { mark(pt, fl, -1, 0); };
   auto Tp = [&](long r, long col) { return Tm + r + col * ldt; };
@@ -482,8 +494,14 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
     const int n2 = n - jn;
     const int jbn = jn < mn ? std::min(nb, mn - jn) : 0;
     const int mrows = m - jn;
-    // the next panel takes XCD 0 for itself when it runs multi-workgroup
-    const bool split = xcdsplit && jbn > 0 && panel_groups(m - jn) > 0 && panel_groups(m - jn) <= 32;
+    // the next panel takes XCD 0 for itself (32 workgroups, one per CU) when
+    // it runs multi-workgroup; the rest of the step leaves it.  A taller
+    // panel (more groups) shares the GPU instead: measured at n = 32768,
+    // reserving a second XCD slows the then dominant update more than it
+    // speeds the panel.
+    const int pg = jbn > 0 ? panel_groups(m - jn) : 0;
+    const int npx = (xcdsplit && pg > 0 && pg <= 32) ? 1 : 0;
+    const bool split = npx > 0;
     // ---- interchanges outside the panel (SRC/dgetrf.f:193,199-200): next panel's columns
     mark(pa, fla, LA_PHASE_LASWP, 0);
     if (j > 0 || jn < n) launch_swaps_to_pairs(j, jn, m, ipiv, pairs, npairs, gmap, s);
@@ -514,17 +532,17 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
     // ---- the rest, beside the next panel: interchanges left of this panel
     //      and right of the next one, their TRSM and update
     mark(pa, fla, LA_PHASE_LASWP, 0);
-    if (j > 0 || jn + jbn < n) K<T>::apply_cols(Tm, ldt, 0, j, jn + jbn, n, pairs, npairs, s, split);
+    if (j > 0 || jn + jbn < n) K<T>::apply_cols(Tm, ldt, 0, j, jn + jbn, n, pairs, npairs, s, npx);
     endm(pa, fla);
     const int nr = n2 - jbn;
     if (nr > 0) {
       mark(pa, fla, LA_PHASE_TRSM, 2.0 * nr * jb * (double)jb / 2.0);
-      K<T>::gemm('N', 'N', nr, jb, jb, T(1), Tp(jn + jbn, j), ldt, W, ldw, T(0), X + jbn, ldx, s, split);
-      K<T>::copy(nr, jb, X + jbn, ldx, Tp(jn + jbn, j), ldt, s, split);
+      K<T>::gemm('N', 'N', nr, jb, jb, T(1), Tp(jn + jbn, j), ldt, W, ldw, T(0), X + jbn, ldx, s, npx);
+      K<T>::copy(nr, jb, X + jbn, ldx, Tp(jn + jbn, j), ldt, s, npx);
       endm(pa, fla);
       if (mrows > 0) {
         mark(pa, fla, LA_PHASE_GEMM, 2.0 * nr * (double)mrows * jb);
-        K<T>::update(nr, mrows, jb, X + jbn, ldx, Tp(j, jn), ldt, Tp(jn + jbn, jn), ldt, split, s);
+        K<T>::update(nr, mrows, jb, X + jbn, ldx, Tp(j, jn), ldt, Tp(jn + jbn, jn), ldt, npx, s);
         endm(pa, fla);
       }
     }
@@ -603,10 +621,10 @@ int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* i
 
   PhaseTimer pt;
   pt.on = g_phase_on.load() != 0;
+  pt.level = g_phase_on.load();
   pt.s = s;
   std::vector<double> fl;
-  pt.mark(LA_PHASE_GETRS);
-  fl.push_back(2.0 * (double)n * n * nrhs);
+  if (pt.mark(LA_PHASE_GETRS)) fl.push_back(2.0 * (double)n * n * nrhs);
 
   if (!getrs_use_gemm()) {
     launch_fill_i32(cerr, 1, 0, s);
@@ -623,8 +641,7 @@ int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* i
       launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s);               // dgetrs.f:217
       K<T>::gather(n, nrhs, perm, X, ldx, B, ldb, s);
     }
-    pt.mark(-1);
-    fl.push_back(0);
+    if (pt.mark(-1)) fl.push_back(0);
     rc = ws_release(c, s);
     phase_collect(pt, fl);
     return rc;
@@ -670,8 +687,7 @@ int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* i
     launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s);              // dgetrs.f:217
     K<T>::gather(n, nrhs, perm, X, ldx, B, ldb, s);
   }
-  pt.mark(-1);
-  fl.push_back(0);
+  if (pt.mark(-1)) fl.push_back(0);
   rc = ws_release(c, s);
   phase_collect(pt, fl);
   return rc;

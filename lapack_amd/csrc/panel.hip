@@ -54,6 +54,9 @@ template <> struct Sfmin<float> { static constexpr float v = 0x1p-126f; };
 
 typedef unsigned long long u64;
 
+// XCDs a rank-mode panel of G workgroups occupies (32 CUs each).
+__host__ __device__ __forceinline__ unsigned panel_xcds(int G) { return (unsigned)((G + 31) / 32); }
+
 // Pivot key: see the header comment.
 template <class T>
 __device__ __forceinline__ u64 pivot_key(T x, bool is_k) {
@@ -85,9 +88,10 @@ __device__ __forceinline__ u64 wave_max_u64(u64 v) {
   return ((u64)hi << 32) | lo;
 }
 
-// XCD0: the grid holds 8*G blocks; only blocks running on XCD 0 take part
-// (ranked in arrival order through *rankctr), so the panel owns that XCD's
-// CUs while a concurrent trailing update keeps to the other seven.
+// XCD0: the grid holds (8 / panel_xcds(G)) * G blocks; only blocks running
+// on the first panel_xcds(G) XCDs take part (ranked in arrival order through
+// *rankctr), so the panel owns those XCDs' CUs (one 512-thread workgroup per
+// CU, 32 per XCD) while a concurrent trailing update keeps to the others.
 template <class T, int W, int R, bool SOLO, int NTH, bool XCD0>
 __global__ __launch_bounds__(NTH) void getf2_panel_kernel(T* __restrict__ Tm, long ldt, int j0, int w, int m,
                                                          int* __restrict__ ipiv, int* __restrict__ info,
@@ -109,7 +113,7 @@ __global__ __launch_bounds__(NTH) void getf2_panel_kernel(T* __restrict__ Tm, lo
   if (XCD0) {
     if (tid == 0) {
       int r = -1;
-      if ((__builtin_amdgcn_s_getreg(6164) & 15u) == 0u)  // HW_REG_XCC_ID
+      if ((__builtin_amdgcn_s_getreg(6164) & 15u) < panel_xcds(Gp))  // HW_REG_XCC_ID
         r = (int)__hip_atomic_fetch_add(rankctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_rank = (r >= 0 && r < Gp) ? r : -1;
     }
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(NTH) void getf2_panel_ll_kernel(T* __restrict__ Tm,
   if (XCD0) {
     if (tid == 0) {
       int r = -1;
-      if ((__builtin_amdgcn_s_getreg(6164) & 15u) == 0u)  // HW_REG_XCC_ID
+      if ((__builtin_amdgcn_s_getreg(6164) & 15u) < panel_xcds(Gp))  // HW_REG_XCC_ID
         r = (int)__hip_atomic_fetch_add(rankctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_rank = (r >= 0 && r < Gp) ? r : -1;
     }
@@ -581,7 +585,7 @@ __global__ __launch_bounds__(NTH) void getf2_panel_ll8_kernel(T* __restrict__ Tm
   if (XCD0) {
     if (tid == 0) {
       int r = -1;
-      if ((__builtin_amdgcn_s_getreg(6164) & 15u) == 0u)  // HW_REG_XCC_ID
+      if ((__builtin_amdgcn_s_getreg(6164) & 15u) < panel_xcds(Gp))  // HW_REG_XCC_ID
         r = (int)__hip_atomic_fetch_add(rankctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_rank = (r >= 0 && r < Gp) ? r : -1;
     }
@@ -810,7 +814,7 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
   if (XCD0) {
     if (tid == 0) {
       int r = -1;
-      if ((__builtin_amdgcn_s_getreg(6164) & 15u) == 0u)  // HW_REG_XCC_ID
+      if ((__builtin_amdgcn_s_getreg(6164) & 15u) < panel_xcds(Gp))  // HW_REG_XCC_ID
         r = (int)__hip_atomic_fetch_add(rankctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_rank = (r >= 0 && r < Gp) ? r : -1;
     }
@@ -1049,8 +1053,8 @@ static int launch_panel_t(T* Tm, long ldt, int j0, int w, int m, int* ipiv, int*
     if (G > GMAX) return LA_ERR_LIMIT;
     if (panel_variant() == 3) {
       u64* xb = reinterpret_cast<u64*>(xchg);
-      if (xcd0 && G <= 32)
-        getf2_panel_w8_kernel<T, 512, true><<<8 * G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase, xb, err,
+      if (xcd0 && G <= 64)
+        getf2_panel_w8_kernel<T, 512, true><<<(8 / (int)panel_xcds(G)) * G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase, xb, err,
                                                                  G, rankctr);
       else
         getf2_panel_w8_kernel<T, 512, false><<<G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase, xb, err, G,
@@ -1059,8 +1063,8 @@ static int launch_panel_t(T* Tm, long ldt, int j0, int w, int m, int* ipiv, int*
     }
     if (panel_variant() == 2) {
       u64* xb = reinterpret_cast<u64*>(xchg);
-      if (xcd0 && G <= 32)
-        getf2_panel_ll8_kernel<T, 512, true><<<8 * G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase, xb, err,
+      if (xcd0 && G <= 64)
+        getf2_panel_ll8_kernel<T, 512, true><<<(8 / (int)panel_xcds(G)) * G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase, xb, err,
                                                                   G, rankctr);
       else
         getf2_panel_ll8_kernel<T, 512, false><<<G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase, xb, err, G,
@@ -1069,16 +1073,16 @@ static int launch_panel_t(T* Tm, long ldt, int j0, int w, int m, int* ipiv, int*
     }
     if (panel_variant() == 1) {
       u64* xb = reinterpret_cast<u64*>(xchg);
-      if (xcd0 && G <= 32)
-        getf2_panel_ll_kernel<T, 64, 512, true><<<8 * G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase, xb,
+      if (xcd0 && G <= 64)
+        getf2_panel_ll_kernel<T, 64, 512, true><<<(8 / (int)panel_xcds(G)) * G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase, xb,
                                                                      err, G, rankctr);
       else
         getf2_panel_ll_kernel<T, 64, 512, false><<<G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase, xb, err,
                                                                   G, rankctr);
       return 0;
     }
-    if (xcd0 && G <= 32)  // one 512-thread workgroup per CU of XCD 0
-      getf2_panel_kernel<T, 64, 1, false, 512, true><<<8 * G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase,
+    if (xcd0 && G <= 64)  // one 512-thread workgroup per CU of XCD 0
+      getf2_panel_kernel<T, 64, 1, false, 512, true><<<(8 / (int)panel_xcds(G)) * G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase,
                                                                           xchg, tags, err, G, rankctr);
     else
       getf2_panel_kernel<T, 64, 1, false, 512, false><<<G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase,
