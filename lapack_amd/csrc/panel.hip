@@ -881,13 +881,15 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
           if (kx > bk) { bk = kx; bw = x; }
         }
         if (wave == bw) {
+          // header first: the readers' first poll needs only the keys, the
+          // row is fetched by their second
           u64* sl = buf + (size_t)b * LL_SLOT;
-          if (bk != 0ull) publish_row(sl, wl, flag, v);
           if (lane == 0) {
-            st_dev(sl + 2, ll_word((unsigned)s_wrow[bw], flag));
             st_dev(sl + 0, ll_word((unsigned)(bk & 0xffffffffull), flag));
             st_dev(sl + 1, ll_word((unsigned)(bk >> 32), flag));
+            st_dev(sl + 2, ll_word((unsigned)s_wrow[bw], flag));
           }
+          if (bk != 0ull) publish_row(sl, wl, flag, v);
         }
         // 3. wave 0: the G candidates and row k, then the winner's row
         if (wave == 0) {
