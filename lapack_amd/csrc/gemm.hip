@@ -53,6 +53,44 @@ template <> struct Mfma<float> {
   static constexpr int APAD = 16;
 };
 
+// The update forms read C into the accumulators before the k loop
+// (acc = C, then acc += (-A) B, the order of BLAS/SRC/dgemm.f:265-276 with
+// alpha = -1), so the epilogue only stores: the 64 loads per lane are in
+// flight together beside the first A/B tile instead of costing one memory
+// round trip each after the loop.  Rows and columns past the edge of C read
+// a clamped in-bounds element and are never stored.
+template <class T, class Acc>
+__device__ __forceinline__ void preload_c(Acc (&acc)[4][4], const T* __restrict__ C, long ldc, int M, int N,
+                                          int r0, int c0, int li, int lk) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const T* cp = C + (long)min(c0 + j * 16 + li, N - 1) * ldc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = cp[min(r0 + i * 16 + Mfma<T>::row(lk, r), M - 1)];
+  }
+}
+
+// Stores scale * acc for the in-bounds elements (no loads).
+template <class T, class Acc>
+__device__ __forceinline__ void store_c(const Acc (&acc)[4][4], T* __restrict__ C, long ldc, int M, int N, int r0,
+                                        int c0, int li, int lk, T scale) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int gc = c0 + j * 16 + li;
+    if (gc >= N) continue;
+    T* cp = C + (long)gc * ldc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = r0 + i * 16 + Mfma<T>::row(lk, r);
+        if (gr < M) cp[gr] = scale == T(1) ? acc[i][j][r] : scale * acc[i][j][r];
+      }
+  }
+}
+
 // UPD: the trailing-update form C <- C - A*B (alpha = -1, beta = 1), a
 // separate symbol so profiles can single out the DGETRF trailing update.
 template <class T, bool TA, bool TB, bool UPD>
@@ -91,11 +129,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(int M, int N, int K, T alph
   const int wm = wave & 1;   // wave row in the 2x2 wave grid
   const int wn = wave >> 1;  // wave column
 
+  const int li = lane & 15;  // row (A) / column (B) inside a 16-block
+  const int lk = lane >> 4;  // k inside a 4-step
   acc_t acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   T ra[8], rb[8];
   const int nkt = (K + BK - 1) / BK;
@@ -129,7 +165,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(int M, int N, int K, T alph
       int i, k;
       if (!TA) { i = tid & 127; k = (tid >> 7) + 2 * q; }
       else     { k = tid & 15;  i = (tid >> 4) + 16 * q; }
-      As[buf][k * AST + i] = ra[q];
+      As[buf][k * AST + i] = UPD ? -ra[q] : ra[q];
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -140,14 +176,20 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(int M, int N, int K, T alph
     }
   };
 
-  if (nkt > 0) {
-    load_tile(0);
-    store_tile(0);
+  // first A/B tile in flight before the C preload, so that its LDS store
+  // waits for those loads only
+  if (nkt > 0) load_tile(0);
+  if (UPD)
+    preload_c<T>(acc, C, ldc, M, N, row0 + wm * 64, col0 + wn * 64, li, lk);
+  else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
   }
+  if (nkt > 0) store_tile(0);
   __syncthreads();
 
-  const int li = lane & 15;  // row (A) / column (B) inside a 16-block
-  const int lk = lane >> 4;  // k inside a 4-step
   for (int kt = 0; kt < nkt; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nkt) load_tile(kt + 1);
@@ -172,23 +214,35 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(int M, int N, int K, T alph
 
   // Epilogue: each lane holds four elements of column lane%16 of every
   // 16x16 block, at the rows given by Mfma<T>::row.
+  if (UPD || beta == T(0)) {
+    store_c<T>(acc, C, ldc, M, N, row0 + wm * 64, col0 + wn * 64, li, lk, UPD ? T(1) : alpha);
+    return;
+  }
+  // beta != 0: a 16-row band of C at a time is read with every load in
+  // flight before the first use (clamped, unconditional addresses), then
+  // combined and stored.
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    T cv[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gc = min(col0 + wn * 64 + j * 16 + li, N - 1);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = min(row0 + wm * 64 + i * 16 + Mfma<T>::row(lk, r), M - 1);
+        cv[j][r] = C[(long)gr + (long)gc * ldc];
+      }
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int gc = col0 + wn * 64 + j * 16 + li;
-      if (gc >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gr = row0 + wm * 64 + i * 16 + Mfma<T>::row(lk, r);
-        if (gr < M) {
-          T* cp = C + (long)gr + (long)gc * ldc;
-          const T v = acc[i][j][r];
-          if (UPD) *cp = *cp - v;
-          else *cp = (beta == T(0)) ? alpha * v : alpha * v + beta * *cp;
-        }
+        if (gr < M && gc < N) C[(long)gr + (long)gc * ldc] = alpha * acc[i][j][r] + beta * cv[j][r];
       }
     }
+  }
 }
 
 // The trailing update C <- C - A*B with A, B, C untransposed, for large
@@ -239,11 +293,9 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
   const int wm = wave % WM;
   const int wn = wave / WM;
 
+  const int li = lane & 15, lk = lane >> 4;
   acc_t acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  preload_c<T>(acc, C, ldc, M, N, row0 + wm * 64, col0 + wn * 64, li, lk);
 
   T ra[LA], rb[LB];
   const int nkt = (K + BK - 1) / BK;
@@ -274,7 +326,7 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
 #pragma unroll
     for (int q = 0; q < LA; ++q) {
       const int e = tid + q * NTB;
-      As[buf][(e / TBM) * AST + e % TBM] = ra[q];
+      As[buf][(e / TBM) * AST + e % TBM] = -ra[q];
     }
 #pragma unroll
     for (int q = 0; q < LB; ++q) {
@@ -282,12 +334,10 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
       Bs[buf][(e / BK) * BST + e % BK] = rb[q];
     }
   };
-  if (nkt > 0) {
-    load_tile(0);
-    store_tile(0);
-  }
+  if (nkt > 0) load_tile(0);
+  preload_c<T>(acc, C, ldc, M, N, row0 + wm * 64, col0 + wn * 64, li, lk);
+  if (nkt > 0) store_tile(0);
   __syncthreads();
-  const int li = lane & 15, lk = lane >> 4;
   for (int kt = 0; kt < nkt; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nkt) load_tile(kt + 1);
@@ -309,21 +359,7 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
     if (kt + 1 < nkt) store_tile(cur ^ 1);
     __syncthreads();
   }
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int gc = col0 + wn * 64 + j * 16 + li;
-      if (gc >= N) continue;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gr = row0 + wm * 64 + i * 16 + Mfma<T>::row(lk, r);
-        if (gr < M) {
-          T* cp = C + (long)gr + (long)gc * ldc;
-          *cp = *cp - acc[i][j][r];
-        }
-      }
-    }
+  store_c<T>(acc, C, ldc, M, N, row0 + wm * 64, col0 + wn * 64, li, lk, T(1));
   }
 }
 

@@ -159,21 +159,30 @@ __global__ __launch_bounds__(HNT, 2) void gemm_update_h_kernel(int M, int N, int
     __syncthreads();
   }
 
+  // A 16-row band of C (16 elements per lane) is loaded with clamped,
+  // unconditional addresses before its first use, so the band's memory
+  // round trips overlap instead of following one another.
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    float cv[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float* cp = C + (long)min(col0 + wn * 64 + j * 16 + li, N - 1) * ldc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cv[j][r] = cp[min(row0 + wm * 64 + i * 16 + 4 * lk + r, M - 1)];
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int gc = col0 + wn * 64 + j * 16 + li;
       if (gc >= N) continue;
+      float* cp = C + (long)gc * ldc;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gr = row0 + wm * 64 + i * 16 + 4 * lk + r;
-        if (gr < M) {
-          float* cp = C + (long)gr + (long)gc * ldc;
-          *cp = *cp - acc[i][j][r] * (1.0f / BSCALE);
-        }
+        if (gr < M) cp[gr] = cv[j][r] - acc[i][j][r] * (1.0f / BSCALE);
       }
     }
+  }
 }
 
 }  // namespace

@@ -3,7 +3,8 @@
 Bars (the reference's own test ratios):
   * IPIV identical to the reference DGETRF (integer output).  A divergence is
     accepted only at a documented near-tie: at the first differing step k the
-    reference's second-best candidate has |L(i,k)| >= 1 - 1e-9.
+    reference's second-best candidate has |L(i,k)| >= 1 - 1e-9 (FP64 factors;
+    FP32 factors use 1 - n * 2^-24, the rounding their entries carry).
   * DGET01 ||P*L*U - A||_1/(n ||A||_1 eps) < 30 (TESTING/LIN/dget01.f, threshold
     TESTING/dtest.in:13), DGET02 ||B - A X||_1/(||A||_1 ||X||_1 eps) < 30.
   * GEMM: DBLAT3's componentwise ratio |ct - cc|/(eps*g) < 16
@@ -37,11 +38,13 @@ def near_tie(lu_ref, k):
     return 1.0 - col.max()
 
 
-def assert_ipiv(ipiv_gpu, ipiv_ref, lu_ref):
+def assert_ipiv(ipiv_gpu, ipiv_ref, lu_ref, tol=1e-9):
+    """tol: the near-tie scale of the factoring precision, 1e-9 for FP64;
+    FP32 factors carry rounding of order n * 2^-24 and pass that."""
     k = first_mismatch(ipiv_gpu, ipiv_ref)
     if k >= 0:
         gap = near_tie(lu_ref, k)
-        assert gap < 1e-9, f"IPIV differs at step {k} without a near-tie (gap {gap:.3e})"
+        assert gap < tol, f"IPIV differs at step {k} without a near-tie (gap {gap:.3e})"
 
 
 def gpu_getrf(a, dtype=np.float64):
