@@ -53,23 +53,31 @@ template <> struct Mfma<float> {
   static constexpr int APAD = 16;
 };
 
+// Products are issued as C^T = B^T A^T (the B fragment in the MFMA's A
+// slot), so block (i, j) of a wave's 64x64 C tile holds, in register r of
+// lane l, C(16i + l%16, 16j + Mfma<T>::row(l/16, r)): for each register the
+// 16 lanes of a group cover 16 consecutive rows of one column, whole
+// 128-byte lines for FP64 and half lines for FP32.  With the operands the
+// other way round every C access instruction touched 16 lines per wave in
+// 32-byte (FP64) or 4-byte (FP32) pieces; measured on MI355X at m = n = 16128,
+// k = 256 the C traffic then ran at 2.0-4.1 TB/s against 5.1 for the
+// coalesced form, 41.1 -> 46.9 TFLOP/s (FP64) and 61.2 -> 84.9 (FP32).
+//
 // The update forms read C into the accumulators before the k loop
 // (acc = C, then acc += (-A) B, the order of BLAS/SRC/dgemm.f:265-276 with
-// alpha = -1), so the epilogue only stores: the 64 loads per lane are in
-// flight together beside the first A/B tile instead of costing one memory
-// round trip each after the loop.  Rows and columns past the edge of C read
-// a clamped in-bounds element and are never stored.
+// alpha = -1), so the epilogue only stores.  Rows and columns past the edge
+// of C read a clamped in-bounds element and are never stored.
 template <class T, class Acc>
 __device__ __forceinline__ void preload_c(Acc (&acc)[4][4], const T* __restrict__ C, long ldc, int M, int N,
                                           int r0, int c0, int li, int lk) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const T* cp = C + (long)min(c0 + j * 16 + li, N - 1) * ldc;
+  for (int j = 0; j < 4; ++j)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int r = 0; r < 4; ++r) {
+      const T* cp = C + (long)min(c0 + j * 16 + Mfma<T>::row(lk, r), N - 1) * ldc;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[i][j][r] = cp[min(r0 + i * 16 + Mfma<T>::row(lk, r), M - 1)];
-  }
+      for (int i = 0; i < 4; ++i) acc[i][j][r] = cp[min(r0 + i * 16 + li, M - 1)];
+    }
 }
 
 // Stores scale * acc for the in-bounds elements (no loads).
@@ -77,18 +85,18 @@ template <class T, class Acc>
 __device__ __forceinline__ void store_c(const Acc (&acc)[4][4], T* __restrict__ C, long ldc, int M, int N, int r0,
                                         int c0, int li, int lk, T scale) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int gc = c0 + j * 16 + li;
-    if (gc >= N) continue;
-    T* cp = C + (long)gc * ldc;
+  for (int j = 0; j < 4; ++j)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int r = 0; r < 4; ++r) {
+      const int gc = c0 + j * 16 + Mfma<T>::row(lk, r);
+      if (gc >= N) continue;
+      T* cp = C + (long)gc * ldc;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gr = r0 + i * 16 + Mfma<T>::row(lk, r);
+      for (int i = 0; i < 4; ++i) {
+        const int gr = r0 + i * 16 + li;
         if (gr < M) cp[gr] = scale == T(1) ? acc[i][j][r] : scale * acc[i][j][r];
       }
-  }
+    }
 }
 
 // UPD: the trailing-update form C <- C - A*B (alpha = -1, beta = 1), a
@@ -206,14 +214,13 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(int M, int N, int K, T alph
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(bf[j], af[i], acc[i][j]);
     }
     if (kt + 1 < nkt) store_tile(cur ^ 1);
     __syncthreads();
   }
 
-  // Epilogue: each lane holds four elements of column lane%16 of every
-  // 16x16 block, at the rows given by Mfma<T>::row.
+  // Epilogue (element layout as described at preload_c).
   if (UPD || beta == T(0)) {
     store_c<T>(acc, C, ldc, M, N, row0 + wm * 64, col0 + wn * 64, li, lk, UPD ? T(1) : alpha);
     return;
@@ -223,25 +230,22 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(int M, int N, int K, T alph
   // combined and stored.
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
+    const int gr = row0 + wm * 64 + i * 16 + li;
     T cv[4][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int gc = min(col0 + wn * 64 + j * 16 + li, N - 1);
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gr = min(row0 + wm * 64 + i * 16 + Mfma<T>::row(lk, r), M - 1);
-        cv[j][r] = C[(long)gr + (long)gc * ldc];
+        const int gc = min(col0 + wn * 64 + j * 16 + Mfma<T>::row(lk, r), N - 1);
+        cv[j][r] = C[(long)min(gr, M - 1) + (long)gc * ldc];
       }
-    }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int gc = col0 + wn * 64 + j * 16 + li;
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gr = row0 + wm * 64 + i * 16 + Mfma<T>::row(lk, r);
+        const int gc = col0 + wn * 64 + j * 16 + Mfma<T>::row(lk, r);
         if (gr < M && gc < N) C[(long)gr + (long)gc * ldc] = alpha * acc[i][j][r] + beta * cv[j][r];
       }
-    }
   }
 }
 
@@ -295,7 +299,6 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
 
   const int li = lane & 15, lk = lane >> 4;
   acc_t acc[4][4];
-  preload_c<T>(acc, C, ldc, M, N, row0 + wm * 64, col0 + wn * 64, li, lk);
 
   T ra[LA], rb[LB];
   const int nkt = (K + BK - 1) / BK;
@@ -354,7 +357,7 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(bf[j], af[i], acc[i][j]);
     }
     if (kt + 1 < nkt) store_tile(cur ^ 1);
     __syncthreads();

@@ -15,6 +15,10 @@
 // in LDS, where one MFMA operand (4 consecutive k of one row/column) is a
 // single 8-byte read.  MFMA layouts: lane l supplies A[l%16][4(l/16)..+3]
 // and B[4(l/16)..+3][l%16]; accumulator r of lane l is D[4(l/16)+r][l%16].
+// Products are issued as C^T = B^T A^T, so register r of lane l of block
+// (i, j) holds C(16i + l%16, 16j + 4(l/16) + r): each C access instruction
+// covers 16 consecutive rows of 4 columns (64-byte pieces) instead of
+// scattered 4-byte pieces of 16 columns.
 #include "la_common.h"
 
 namespace la {
@@ -153,7 +157,7 @@ __global__ __launch_bounds__(HNT, 2) void gemm_update_h_kernel(int M, int N, int
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = mfma_h<KIND>(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma_h<KIND>(bf[j], af[i], acc[i][j]);
     }
     if (kt + 1 < nkt) store_tile(cur ^ 1);
     __syncthreads();
@@ -164,24 +168,21 @@ __global__ __launch_bounds__(HNT, 2) void gemm_update_h_kernel(int M, int N, int
   // round trips overlap instead of following one another.
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
+    const int gr = row0 + wm * 64 + i * 16 + li;
+    const int grc = min(gr, M - 1);
     float cv[4][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const float* cp = C + (long)min(col0 + wn * 64 + j * 16 + li, N - 1) * ldc;
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) cv[j][r] = cp[min(row0 + wm * 64 + i * 16 + 4 * lk + r, M - 1)];
-    }
+      for (int r = 0; r < 4; ++r) cv[j][r] = C[(long)grc + (long)min(col0 + wn * 64 + j * 16 + 4 * lk + r, N - 1) * ldc];
+    if (gr >= M) continue;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int gc = col0 + wn * 64 + j * 16 + li;
-      if (gc >= N) continue;
-      float* cp = C + (long)gc * ldc;
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gr = row0 + wm * 64 + i * 16 + 4 * lk + r;
-        if (gr < M) cp[gr] = cv[j][r] - acc[i][j][r] * (1.0f / BSCALE);
+        const int gc = col0 + wn * 64 + j * 16 + 4 * lk + r;
+        if (gc < N) C[(long)gr + (long)gc * ldc] = cv[j][r] - acc[i][j][r] * (1.0f / BSCALE);
       }
-    }
   }
 }
 
