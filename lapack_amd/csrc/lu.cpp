@@ -48,7 +48,7 @@ struct DevCtx {
   bool ws_used = false;
   hipStream_t pstream = nullptr;      // panel stream (look-ahead)
   hipStream_t ustream = nullptr;      // trailing-update stream
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   char* mp = nullptr;            // buffers of the mixed-precision solvers
   size_t mp_bytes = 0;
   std::mutex mu;     // guards ws and the launch sequence of one call
@@ -390,7 +390,7 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
       LA_TRY(hipStreamCreateWithFlags(&c->pstream, hipStreamNonBlocking));
       LA_TRY(hipStreamCreateWithFlags(&c->ustream, hipStreamNonBlocking));
     }
-    for (int i = 0; i < 4; ++i) LA_TRY(hipEventCreateWithFlags(&c->ev[i], hipEventDisableTiming));
+    for (int i = 0; i < 5; ++i) LA_TRY(hipEventCreateWithFlags(&c->ev[i], hipEventDisableTiming));
   }
   hipStream_t sp = c->pstream;
   hipStream_t s_user = s;
@@ -477,13 +477,21 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
     return 0;
   };
 
+  // W = inv(L11^T) of panel jp (the DTRSM's triangle) into its buffer;
+  // buffers alternate between consecutive panels
+  auto inv_tri = [&](int jp, int jbp, hipStream_t st, PhaseTimer& pt, std::vector<double>& fl) {
+    mark(pt, fl, LA_PHASE_TRSM, 0);
+    K<T>::trtri_blocks(true, true, jbp, jbp, Tp(jp, jp), ldt, Ws[(jp / nb) & 1], ldw, st);
+    endm(pt, fl);
+  };
   // panel 0 on the main stream
   rc = factor_panel(0, std::min(nb, mn), s, pa, fla, false);
+  if (rc == 0 && n > std::min(nb, mn)) inv_tri(0, std::min(nb, mn), s, pa, fla);
   for (int j = 0; j < mn && rc == 0; j += nb) {
     const int jb = std::min(nb, mn - j);
     if (j > 0) LA_TRY(hipStreamWaitEvent(s, c->ev[0], 0));  // panel j done (panel stream)
     T* X = Xs[0];
-    T* W = Ws[0];
+    T* W = Ws[(j / nb) & 1];
     int* pairs = Ps[0];
     int* npairs = pairs + 2 * swp_max_pairs();
     // Only the next panel's columns are on the critical path: their
@@ -507,14 +515,13 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
     if (j > 0 || jn < n) launch_swaps_to_pairs(j, jn, m, ipiv, pairs, npairs, gmap, s);
     if (jbn > 0) K<T>::apply_cols(Tm, ldt, jn, jn + jbn, 0, 0, pairs, npairs, s);
     endm(pa, fla);
-    if (n2 > 0) {
+    // the triangle's inverse, computed on the panel stream after panel j
+    if (j > 0 && n2 > 0) LA_TRY(hipStreamWaitEvent(s, c->ev[4], 0));
+    if (jbn > 0) {
       // ---- DTRSM: U12^T = A12^T * inv(L11^T)  (SRC/dgetrf.f:204-207), next panel's columns
+      //      into X (copied back into T after that panel has been released)
       mark(pa, fla, LA_PHASE_TRSM, 2.0 * jbn * jb * (double)jb / 2.0);
-      K<T>::trtri_blocks(true, true, jb, jb, Tp(j, j), ldt, W, ldw, s);
-      if (jbn > 0) {
-        K<T>::gemm('N', 'N', jbn, jb, jb, T(1), Tp(jn, j), ldt, W, ldw, T(0), X, ldx, s);
-        K<T>::copy(jbn, jb, X, ldx, Tp(jn, j), ldt, s);
-      }
+      K<T>::gemm('N', 'N', jbn, jb, jb, T(1), Tp(jn, j), ldt, W, ldw, T(0), X, ldx, s);
       endm(pa, fla);
     }
     // ---- DGEMM: A22 -= A21*U12, i.e. T22 -= T12 * T21 (SRC/dgetrf.f:212-216),
@@ -528,6 +535,14 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
       rc = factor_panel(jn, jbn, sp, pp, flp, split);
       if (rc) break;
       LA_TRY(hipEventRecord(c->ev[0], sp));
+      // its triangle's inverse, beside the next step's interchanges
+      if (n > jn + jbn) inv_tri(jn, jbn, sp, pp, flp);
+      LA_TRY(hipEventRecord(c->ev[4], sp));
+    }
+    if (jbn > 0) {
+      mark(pa, fla, LA_PHASE_TRSM, 0);
+      K<T>::copy(jbn, jb, X, ldx, Tp(jn, j), ldt, s, npx);
+      endm(pa, fla);
     }
     // ---- the rest, beside the next panel: interchanges left of this panel
     //      and right of the next one, their TRSM and update
