@@ -787,9 +787,10 @@ __global__ __launch_bounds__(NTH) void getf2_panel_ll8_kernel(T* __restrict__ Tm
 //   B1  every wave leaves (key, row) in LDS; all threads fold the eight;
 //       the winning wave stages its winning lane's row in LDS and stores
 //       it, flag-tagged, to the workgroup's slot (no barrier, no wait);
-//   wave 0 polls the G slot headers and row k (published at the start of
-//       the column by the wave holding it), takes the max key / lowest slot,
-//       loads the winner's row (flag-validated), hands it over through LDS;
+//   the wave holding row k publishes it after the candidate;
+//   wave 0 polls the G slot headers, takes the max key / lowest slot, loads
+//       the winner's row and row k (flag-validated), hands them over
+//       through LDS;
 //   B2  all threads interchange, scale and update.
 // LDS hand-over buffers alternate with the column parity, so a wave may run
 // one column ahead without a barrier at the end of each column.
@@ -861,7 +862,6 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
         const int par = k & 1;
         const unsigned flag = tagbase + (unsigned)k + 1u;
         u64* buf = xbuf + (size_t)par * LL_PAR;
-        if (k >= wrow0 && k < wrow0 + 64) publish_row(buf + (size_t)LL_SMAX * LL_SLOT, k - wrow0, flag, v);
         // 1. wave candidates -> LDS
         const bool act = lr >= k && lr < h;
         const u64 key = act ? pivot_key(v[kk], lr == k) : 0ull;
@@ -891,7 +891,13 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
           }
           if (bk != 0ull) publish_row(sl, wl, flag, v);
         }
-        // 3. wave 0: the G candidates and row k, then the winner's row
+        // row k, for the interchange, after the candidate: only the owner of
+        // the pivot row needs it, once the exchange is over, so it stays off
+        // the path to the workgroup's header (measured: publishing it first
+        // made the workgroup holding the panel's top rows the last to
+        // publish in every column)
+        if (k >= wrow0 && k < wrow0 + 64) publish_row(buf + (size_t)LL_SMAX * LL_SLOT, k - wrow0, flag, v);
+        // 3. wave 0: the G candidates, then the winner's row and row k
         if (wave == 0) {
           u64 kv = 0ull;
           bool kok = lane >= G;
@@ -908,8 +914,7 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
                 kok = true;
               }
             }
-            if (!rk_ok) rk_ok = ll_get_double(rks + 4 + 2 * lane, flag, &rkv);
-            if (__all(kok && rk_ok)) break;
+            if (__all(kok)) break;
             __builtin_amdgcn_s_sleep(1);
             ++spins;
             if ((spins & 255u) == 0 && ld_dev(err) != 0) failed = true;
@@ -930,7 +935,8 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
               roww = ld_dev(src + 2);
               rok = (unsigned)(roww >> 32) == flag;
             }
-            if (__all(ok && rok)) break;
+            if (!rk_ok) rk_ok = ll_get_double(rks + 4 + 2 * lane, flag, &rkv);
+            if (__all(ok && rok && rk_ok)) break;
             __builtin_amdgcn_s_sleep(1);
             ++spins;
             if ((spins & 255u) == 0 && ld_dev(err) != 0) failed = true;
