@@ -366,6 +366,95 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
   }
 }
 
+// Small products (C = alpha*A*B + beta*C, A and B untransposed, few 128x128
+// tiles: the DTRSM product of the next panel's 256 columns is 256x256x256,
+// four such tiles): 64x64 tiles, so that 16 workgroups share the work.  A
+// 256-thread workgroup = 2x2 waves of 2x2 MFMA blocks, BK = 16, double-
+// buffered LDS, products issued as C^T = B^T A^T as in gemm_kernel.
+template <class T>
+__global__ __launch_bounds__(256) void gemm64_kernel(int M, int N, int K, T alpha, const T* __restrict__ A,
+                                                     long lda, const T* __restrict__ B, long ldb, T beta,
+                                                     T* __restrict__ C, long ldc, int tm) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int TS = 64;
+  constexpr int AST = TS + Mfma<T>::APAD;
+  constexpr int BST = BK + 1;
+  __shared__ T As[2][BK * AST];
+  __shared__ T Bs[2][TS * BST];
+  const int row0 = (blockIdx.x % tm) * TS;
+  const int col0 = (blockIdx.x / tm) * TS;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lk = lane >> 4;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  T ra[4], rb[4];
+  const int nkt = (K + BK - 1) / BK;
+  auto load_tile = [&](int kt) {
+    const int k0 = kt * BK;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = tid & 63, kk = (tid >> 6) + 4 * q;
+      const int gi = row0 + i, gk = k0 + kk;
+      ra[q] = (gi < M && gk < K) ? A[(long)gi + (long)gk * lda] : T(0);
+      const int bk = tid & 15, j = (tid >> 4) + 16 * q;
+      const int gj = col0 + j, gk2 = k0 + bk;
+      rb[q] = (gj < N && gk2 < K) ? B[(long)gk2 + (long)gj * ldb] : T(0);
+    }
+  };
+  auto store_tile = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      As[buf][((tid >> 6) + 4 * q) * AST + (tid & 63)] = ra[q];
+      Bs[buf][((tid >> 4) + 16 * q) * BST + (tid & 15)] = rb[q];
+    }
+  };
+  if (nkt > 0) {
+    load_tile(0);
+    store_tile(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nkt; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nkt) load_tile(kt + 1);
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+      T af[2], bf[2];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        af[b] = As[cur][(kk * 4 + lk) * AST + wm * 32 + b * 16 + li];
+        bf[b] = Bs[cur][(wn * 32 + b * 16 + li) * BST + kk * 4 + lk];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::op(bf[j], af[i], acc[i][j]);
+    }
+    if (kt + 1 < nkt) store_tile(cur ^ 1);
+    __syncthreads();
+  }
+  // register r of lane l of block (i, j): C(16i + l%16, 16j + row(l/16, r))
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int gr = row0 + wm * 32 + i * 16 + li;
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gc = col0 + wn * 32 + j * 16 + Mfma<T>::row(lk, r);
+        if (gr < M && gc < N) {
+          T* cp = C + (long)gr + (long)gc * ldc;
+          *cp = (beta == T(0)) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * *cp;
+        }
+      }
+  }
+}
+
 template <class T>
 void launch_gemm_t(char ta, char tb, int m, int n, int k, T alpha, const T* A, long lda, const T* B,
                    long ldb, T beta, T* C, long ldc, hipStream_t s, int nskip) {
@@ -380,6 +469,11 @@ void launch_gemm_t(char ta, char tb, int m, int n, int k, T alpha, const T* A, l
   const bool TA = (ta == 'T' || ta == 't' || ta == 'C' || ta == 'c');
   const bool TB = (tb == 'T' || tb == 't' || tb == 'C' || tb == 'c');
   if (k <= 0 || alpha == T(0)) k = 0;  // C <- beta*C (dgemm.f:303-318)
+  if (!TA && !TB && nskip == 0 && tiles < 64 && k >= 64) {
+    const int tm64 = (m + 63) / 64, tn64 = (n + 63) / 64;
+    gemm64_kernel<T><<<dim3(tm64 * tn64), 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm64);
+    return;
+  }
   if (!TA && !TB)
     gemm_kernel<T, false, false, false><<<grid, NT, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, chunk, nx);
   else if (!TA && TB)
