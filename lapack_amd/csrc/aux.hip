@@ -208,10 +208,24 @@ __global__ __launch_bounds__(256) void apply_pairs_cols_kernel(T* __restrict__ T
   const int qg = threadIdx.x / SWP_ROWS;  // 16 pair lanes
   const long c = c0 + cl;
   const bool ok = c < cend;
-  for (int q = qg; q < np; q += 256 / SWP_ROWS)
-    if (ok) st[q * SWP_ROWS + cl] = Tm[c + (long)pairs[2 * q + 1] * ldt];
+  // eight sources in flight per thread before they go to LDS (one memory
+  // round trip per eight pairs instead of one per pair)
+  constexpr int QS = 256 / SWP_ROWS, U = 8;
+  for (int q0 = qg; q0 < np; q0 += QS * U) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = q0 + QS * u;
+      v[u] = (ok && q < np) ? Tm[c + (long)pairs[2 * q + 1] * ldt] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = q0 + QS * u;
+      if (ok && q < np) st[q * SWP_ROWS + cl] = v[u];
+    }
+  }
   __syncthreads();
-  for (int q = qg; q < np; q += 256 / SWP_ROWS)
+  for (int q = qg; q < np; q += QS)
     if (ok) Tm[c + (long)pairs[2 * q] * ldt] = st[q * SWP_ROWS + cl];
 }
 
@@ -328,8 +342,20 @@ __global__ __launch_bounds__(256) void apply_pairs_rows_kernel(T* __restrict__ B
   const int j = blockIdx.x * SWP_ROWS + (threadIdx.x & (SWP_ROWS - 1));
   const int qg = threadIdx.x / SWP_ROWS;
   const bool ok = j < ncols;
-  for (int q = qg; q < np; q += 256 / SWP_ROWS)
-    if (ok) st[q * SWP_ROWS + (threadIdx.x & (SWP_ROWS - 1))] = B[(long)pairs[2 * q + 1] + (long)j * ldb];
+  constexpr int QS = 256 / SWP_ROWS, U = 8;
+  for (int q0 = qg; q0 < np; q0 += QS * U) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = q0 + QS * u;
+      v[u] = (ok && q < np) ? B[(long)pairs[2 * q + 1] + (long)j * ldb] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = q0 + QS * u;
+      if (ok && q < np) st[q * SWP_ROWS + (threadIdx.x & (SWP_ROWS - 1))] = v[u];
+    }
+  }
   __syncthreads();
   for (int q = qg; q < np; q += 256 / SWP_ROWS)
     if (ok) B[(long)pairs[2 * q] + (long)j * ldb] = st[q * SWP_ROWS + (threadIdx.x & (SWP_ROWS - 1))];
