@@ -191,7 +191,6 @@ __global__ __launch_bounds__(256) void apply_pairs_cols_kernel(T* __restrict__ T
                                                                int r3, const int* __restrict__ pairs,
                                                                const int* __restrict__ npairs, int nblk,
                                                                int nskip) {
-  __shared__ T st[SWP_MAXPAIRS * SWP_ROWS];
   // nskip: blocks dispatched to XCDs 0 .. nskip-1 (blockIdx % 8) leave at once
   int blk = blockIdx.x;
   if (nskip) {
@@ -200,6 +199,7 @@ __global__ __launch_bounds__(256) void apply_pairs_cols_kernel(T* __restrict__ T
   }
   if (blk >= nblk) return;
   const int np = *npairs;
+  if (np <= 0) return;
   // this block's 16 rows: first from [r0, r1), then from [r2, r3)
   const int n1 = (r1 - r0 + SWP_ROWS - 1) / SWP_ROWS;
   const long c0 = blk < n1 ? (long)r0 + (long)blk * SWP_ROWS : (long)r2 + (long)(blk - n1) * SWP_ROWS;
@@ -208,25 +208,26 @@ __global__ __launch_bounds__(256) void apply_pairs_cols_kernel(T* __restrict__ T
   const int qg = threadIdx.x / SWP_ROWS;  // 16 pair lanes
   const long c = c0 + cl;
   const bool ok = c < cend;
-  // eight sources in flight per thread before they go to LDS (one memory
-  // round trip per eight pairs instead of one per pair)
-  constexpr int QS = 256 / SWP_ROWS, U = 8;
-  for (int q0 = qg; q0 < np; q0 += QS * U) {
-    T v[U];
+  // every pair of the thread (at most SWP_MAXPAIRS / 16), then every source
+  // in registers, all in flight together; the loads complete and one barrier
+  // (a destination may be another pair's source); then every destination
+  constexpr int QS = 256 / SWP_ROWS, U = SWP_MAXPAIRS / QS;
+  int src[U], dst[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int q = q0 + QS * u;
-      v[u] = (ok && q < np) ? Tm[c + (long)pairs[2 * q + 1] * ldt] : T(0);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int q = q0 + QS * u;
-      if (ok && q < np) st[q * SWP_ROWS + cl] = v[u];
-    }
+  for (int u = 0; u < U; ++u) {
+    const int q = min(qg + QS * u, np - 1);
+    src[u] = pairs[2 * q + 1];
+    dst[u] = pairs[2 * q];
   }
+  wait_vm();  // all indices here: the source loads below then go out back to back
+  T v[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) v[u] = (ok && qg + QS * u < np) ? Tm[c + (long)src[u] * ldt] : T(0);
+  wait_vm();
   __syncthreads();
-  for (int q = qg; q < np; q += QS)
-    if (ok) Tm[c + (long)pairs[2 * q] * ldt] = st[q * SWP_ROWS + cl];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (ok && qg + QS * u < np) Tm[c + (long)dst[u] * ldt] = v[u];
 }
 
 // Everything between two leaves of a panel in one single-workgroup launch
@@ -337,28 +338,30 @@ template <class T>
 __global__ __launch_bounds__(256) void apply_pairs_rows_kernel(T* __restrict__ B, long ldb, int ncols,
                                                                const int* __restrict__ pairs,
                                                                const int* __restrict__ npairs) {
-  __shared__ T st[SWP_MAXPAIRS * SWP_ROWS];
   const int np = *npairs;
+  if (np <= 0) return;
   const int j = blockIdx.x * SWP_ROWS + (threadIdx.x & (SWP_ROWS - 1));
   const int qg = threadIdx.x / SWP_ROWS;
   const bool ok = j < ncols;
-  constexpr int QS = 256 / SWP_ROWS, U = 8;
-  for (int q0 = qg; q0 < np; q0 += QS * U) {
-    T v[U];
+  // as apply_pairs_cols_kernel: pairs, then sources in registers, one
+  // barrier, every destination
+  constexpr int QS = 256 / SWP_ROWS, U = SWP_MAXPAIRS / QS;
+  int src[U], dst[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int q = q0 + QS * u;
-      v[u] = (ok && q < np) ? B[(long)pairs[2 * q + 1] + (long)j * ldb] : T(0);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int q = q0 + QS * u;
-      if (ok && q < np) st[q * SWP_ROWS + (threadIdx.x & (SWP_ROWS - 1))] = v[u];
-    }
+  for (int u = 0; u < U; ++u) {
+    const int q = min(qg + QS * u, np - 1);
+    src[u] = pairs[2 * q + 1];
+    dst[u] = pairs[2 * q];
   }
+  wait_vm();
+  T v[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) v[u] = (ok && qg + QS * u < np) ? B[(long)src[u] + (long)j * ldb] : T(0);
+  wait_vm();
   __syncthreads();
-  for (int q = qg; q < np; q += 256 / SWP_ROWS)
-    if (ok) B[(long)pairs[2 * q] + (long)j * ldb] = st[q * SWP_ROWS + (threadIdx.x & (SWP_ROWS - 1))];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (ok && qg + QS * u < np) B[(long)dst[u] + (long)j * ldb] = v[u];
 }
 
 // ---- DLASWP on a column-major matrix (row interchanges) ----------------------
