@@ -510,6 +510,13 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
     const int pg = jbn > 0 ? panel_groups(m - jn) : 0;
     const int npx = (xcdsplit && pg > 0 && pg <= 32) ? 1 : 0;
     const bool split = npx > 0;
+    // While the panel holds (nearly) all of XCD 0, every launch beside it
+    // waits for slots there even for the workgroups that only leave: in the
+    // first steps (measured: 28 or more panel workgroups) the rest-of-matrix
+    // DTRSM product then took 5x its time.  There the rest's interchanges,
+    // DTRSM and copy run before the panel, on the whole GPU (those steps are
+    // bound by the trailing update, not by the panel chain).
+    const bool pre = split && pg >= 27;
     // ---- interchanges outside the panel (SRC/dgetrf.f:193,199-200): next panel's columns
     mark(pa, fla, LA_PHASE_LASWP, 0);
     if (j > 0 || jn < n) launch_swaps_to_pairs(j, jn, m, ipiv, pairs, npairs, gmap, s);
@@ -526,10 +533,25 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
     }
     // ---- DGEMM: A22 -= A21*U12, i.e. T22 -= T12 * T21 (SRC/dgetrf.f:212-216),
     //      the next panel's columns, then that panel on the panel stream
+    const int nr = n2 - jbn;
+    // ---- the rest's interchanges (left of this panel, right of the next one)
+    //      and DTRSM: here, before the next panel, or beside it below
+    auto rest_swap_trsm = [&](int nsk) {
+      mark(pa, fla, LA_PHASE_LASWP, 0);
+      if (j > 0 || jn + jbn < n) K<T>::apply_cols(Tm, ldt, 0, j, jn + jbn, n, pairs, npairs, s, nsk);
+      endm(pa, fla);
+      if (nr > 0) {
+        mark(pa, fla, LA_PHASE_TRSM, 2.0 * nr * jb * (double)jb / 2.0);
+        K<T>::gemm('N', 'N', nr, jb, jb, T(1), Tp(jn + jbn, j), ldt, W, ldw, T(0), X + jbn, ldx, s, nsk);
+        K<T>::copy(nr, jb, X + jbn, ldx, Tp(jn + jbn, j), ldt, s, nsk);
+        endm(pa, fla);
+      }
+    };
     if (jbn > 0 && mrows > 0) {
       mark(pa, fla, LA_PHASE_GEMM, 2.0 * jbn * (double)mrows * jb);
       K<T>::update(jbn, mrows, jb, X, ldx, Tp(j, jn), ldt, Tp(jn, jn), ldt, false, s);
       endm(pa, fla);
+      if (pre) rest_swap_trsm(0);
       LA_TRY(hipEventRecord(c->ev[1], s));
       LA_TRY(hipStreamWaitEvent(sp, c->ev[1], 0));
       rc = factor_panel(jn, jbn, sp, pp, flp, split);
@@ -546,15 +568,8 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
     }
     // ---- the rest, beside the next panel: interchanges left of this panel
     //      and right of the next one, their TRSM and update
-    mark(pa, fla, LA_PHASE_LASWP, 0);
-    if (j > 0 || jn + jbn < n) K<T>::apply_cols(Tm, ldt, 0, j, jn + jbn, n, pairs, npairs, s, npx);
-    endm(pa, fla);
-    const int nr = n2 - jbn;
+    if (!(pre && jbn > 0 && mrows > 0)) rest_swap_trsm(npx);
     if (nr > 0) {
-      mark(pa, fla, LA_PHASE_TRSM, 2.0 * nr * jb * (double)jb / 2.0);
-      K<T>::gemm('N', 'N', nr, jb, jb, T(1), Tp(jn + jbn, j), ldt, W, ldw, T(0), X + jbn, ldx, s, npx);
-      K<T>::copy(nr, jb, X + jbn, ldx, Tp(jn + jbn, j), ldt, s, npx);
-      endm(pa, fla);
       if (mrows > 0) {
         mark(pa, fla, LA_PHASE_GEMM, 2.0 * nr * (double)mrows * jb);
         K<T>::update(nr, mrows, jb, X + jbn, ldx, Tp(j, jn), ldt, Tp(jn + jbn, jn), ldt, npx, s);
