@@ -417,6 +417,8 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
   unsigned* rankctr = reinterpret_cast<unsigned*>(ws + cv.take<unsigned>(nrank));
   const long ldw = nb;
   const bool xcdsplit = env_int("LAPACK_AMD_PANEL_XCD", 1) != 0;
+  // panel workgroups from which the rest's interchanges and DTRSM go first
+  const int pre_pg = env_int("LAPACK_AMD_PRE_PG", 27);
 
   PhaseTimer pa, pp;  // main stream, panel stream
   pa.on = pp.on = g_phase_on.load() != 0;
@@ -516,7 +518,7 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
     // DTRSM product then took 5x its time.  There the rest's interchanges,
     // DTRSM and copy run before the panel, on the whole GPU (those steps are
     // bound by the trailing update, not by the panel chain).
-    const bool pre = split && pg >= 27;
+    const bool pre = split && pg >= pre_pg;
     // ---- interchanges outside the panel (SRC/dgetrf.f:193,199-200): next panel's columns
     mark(pa, fla, LA_PHASE_LASWP, 0);
     if (j > 0 || jn < n) launch_swaps_to_pairs(j, jn, m, ipiv, pairs, npairs, gmap, s);
