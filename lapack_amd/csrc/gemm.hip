@@ -438,7 +438,20 @@ __global__ __launch_bounds__(256) void gemm64_kernel(int M, int N, int K, T alph
     if (kt + 1 < nkt) store_tile(cur ^ 1);
     __syncthreads();
   }
-  // register r of lane l of block (i, j): C(16i + l%16, 16j + row(l/16, r))
+  // register r of lane l of block (i, j): C(16i + l%16, 16j + row(l/16, r));
+  // with beta != 0 the 16 elements of C a lane owns are read first, all in
+  // flight together (clamped addresses), then combined and stored
+  T cv[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = min(row0 + wm * 32 + i * 16 + li, M - 1);
+        const int gc = min(col0 + wn * 32 + j * 16 + Mfma<T>::row(lk, r), N - 1);
+        cv[i][j][r] = beta == T(0) ? T(0) : C[(long)gr + (long)gc * ldc];
+      }
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int gr = row0 + wm * 32 + i * 16 + li;
@@ -447,10 +460,8 @@ __global__ __launch_bounds__(256) void gemm64_kernel(int M, int N, int K, T alph
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gc = col0 + wn * 32 + j * 16 + Mfma<T>::row(lk, r);
-        if (gr < M && gc < N) {
-          T* cp = C + (long)gr + (long)gc * ldc;
-          *cp = (beta == T(0)) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * *cp;
-        }
+        if (gr < M && gc < N)
+          C[(long)gr + (long)gc * ldc] = (beta == T(0)) ? alpha * acc[i][j][r] : alpha * acc[i][j][r] + beta * cv[i][j][r];
       }
   }
 }
@@ -513,6 +524,13 @@ void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long
     if (nskip > 0 && 2.0 * m * (double)n * k < 6e10) per = std::min(chunk, update_cus_per_xcd());
     update_big_kernel<T, 4, 2><<<dim3(per * NXCD), 512, 0, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tmb, tnb, chunk,
                                                                nx);
+    return;
+  }
+  if (m <= 64 && nskip == 0) {
+    // narrow (one leaf's columns): 64x64 tiles fill the GPU, 128x128 would
+    // leave half of every tile empty
+    const int tm64 = (m + 63) / 64, tn64 = (n + 63) / 64;
+    gemm64_kernel<T><<<dim3(tm64 * tn64), 256, 0, s>>>(m, n, k, T(-1), A, lda, B, ldb, T(1), C, ldc, tm64);
     return;
   }
   const int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
