@@ -491,7 +491,9 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(int nrows, int ncols, 
 template <class T, bool UPPER, bool UNIT>
 __global__ __launch_bounds__(256) void trtri_kernel(int n, int nb, const T* __restrict__ U, long ldu,
                                                     T* __restrict__ W, long ldw) {
-  __shared__ T xs;
+  // the solved value of step t goes through xs[t & 1]: one barrier per step
+  // (a slot is rewritten two steps later, after every thread has read it)
+  __shared__ T xs[2];
   const int c = blockIdx.x;
   const int d = blockIdx.y;
   const int b0 = d * nb;
@@ -501,27 +503,49 @@ __global__ __launch_bounds__(256) void trtri_kernel(int n, int nb, const T* __re
   T* Wb = W + (long)d * nb * ldw;  // W block d: nb x nb at column offset d*nb
   const int s = threadIdx.x;
   T x = (s == c) ? T(1) : T(0);
+  // the steps in groups of D, each group's triangle entries loaded first
+  constexpr int D = 8;
   if (UPPER) {
-    for (int t = c; t >= 0; --t) {
-      if (s == t) {
-        if (!UNIT) x = x / Ub[(long)t + (long)t * ldu];
-        xs = x;
+    for (int t0 = c; t0 >= 0; t0 -= D) {
+      T u[D];
+#pragma unroll
+      for (int e = 0; e < D; ++e) {
+        const int t = t0 - e;
+        u[e] = (t >= 0 && s < t) ? Ub[(long)s + (long)t * ldu] : T(0);
       }
-      __syncthreads();
-      const T xt = xs;
-      if (s < t) x = x - xt * Ub[(long)s + (long)t * ldu];
-      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < D; ++e) {
+        const int t = t0 - e;
+        if (t < 0) break;
+        if (s == t) {
+          if (!UNIT) x = x / Ub[(long)t + (long)t * ldu];
+          xs[t & 1] = x;
+        }
+        __syncthreads();
+        const T xt = xs[t & 1];
+        if (s < t) x = x - xt * u[e];
+      }
     }
   } else {
-    for (int t = c; t < bs; ++t) {
-      if (s == t) {
-        if (!UNIT) x = x / Ub[(long)t + (long)t * ldu];
-        xs = x;
+    for (int t0 = c; t0 < bs; t0 += D) {
+      T u[D];
+#pragma unroll
+      for (int e = 0; e < D; ++e) {
+        const int t = t0 + e;
+        u[e] = (t < bs && s > t && s < bs) ? Ub[(long)s + (long)t * ldu] : T(0);
       }
-      __syncthreads();
-      const T xt = xs;
-      if (s > t && s < bs) x = x - xt * Ub[(long)s + (long)t * ldu];
-      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < D; ++e) {
+        const int t = t0 + e;
+        if (t >= bs) break;
+        if (s == t) {
+          if (!UNIT) x = x / Ub[(long)t + (long)t * ldu];
+          xs[t & 1] = x;
+        }
+        __syncthreads();
+        const T xt = xs[t & 1];
+        if (s > t && s < bs) x = x - xt * u[e];
+      }
     }
   }
   if (s < bs) Wb[(long)s + (long)c * ldw] = x;
