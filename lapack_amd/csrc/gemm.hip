@@ -526,14 +526,15 @@ void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long
                                                                nx);
     return;
   }
-  if (m <= 64 && nskip == 0) {
-    // narrow (one leaf's columns): 64x64 tiles fill the GPU, 128x128 would
-    // leave half of every tile empty
+  const int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+  if (nskip == 0 && (m <= 64 || tm * tn < 128)) {
+    // narrow (one leaf's columns) or too few 128x128 tiles to occupy half
+    // the CUs (the next panel's columns late in the factorization): 64x64
+    // tiles spread the same work over four times the workgroups
     const int tm64 = (m + 63) / 64, tn64 = (n + 63) / 64;
     gemm64_kernel<T><<<dim3(tm64 * tn64), 256, 0, s>>>(m, n, k, T(-1), A, lda, B, ldb, T(1), C, ldc, tm64);
     return;
   }
-  const int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
   const int chunk = (tm * tn + nx - 1) / nx;
   gemm_kernel<T, false, false, true><<<dim3(chunk * NXCD), NT, 0, s>>>(m, n, k, T(-1), A, lda, B, ldb, T(1), C, ldc,
                                                                      tm, tn, chunk, nx);
