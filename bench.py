@@ -274,7 +274,8 @@ def main():
         "roofline": {
             "bound": "mfma",
             "kernel": ("DGETRF trailing update A22 -= A21*U12: update_big_kernel<double,4,2> (256x128 tiles), "
-                       "gemm_kernel<double,N,N,update> for the next panel's columns"),
+                       "gemm64_kernel<double,true> (64x64 tiles) for the next panel's columns, "
+                       "gemm_kernel<double,N,N,update> (128x128) for the last updates beside the panel"),
             "achieved": round(achieved, 3),
             "peak": PEAK_FP64,
             "unit": "TFLOP/s",

@@ -371,10 +371,16 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
 // four such tiles): 64x64 tiles, so that 16 workgroups share the work.  A
 // 256-thread workgroup = 2x2 waves of 2x2 MFMA blocks, BK = 16, double-
 // buffered LDS, products issued as C^T = B^T A^T as in gemm_kernel.
-template <class T>
+// UPD: the trailing-update instance (alpha = -1, beta = 1), a symbol of its
+// own so that traces and counters tell it from the DTRSM products.
+template <class T, bool UPD = false>
 __global__ __launch_bounds__(256) void gemm64_kernel(int M, int N, int K, T alpha, const T* __restrict__ A,
                                                      long lda, const T* __restrict__ B, long ldb, T beta,
                                                      T* __restrict__ C, long ldc, int tm) {
+  if constexpr (UPD) {
+    alpha = T(-1);
+    beta = T(1);
+  }
   typedef typename Mfma<T>::acc_t acc_t;
   constexpr int TS = 64;
   constexpr int AST = TS + Mfma<T>::APAD;
@@ -507,6 +513,19 @@ static int update_cus_per_xcd() {
   return v;
 }
 
+// Updates with fewer 128x128 tiles than this (and no XCD left to a panel)
+// take 64x64 tiles; at n = 16384 any threshold from 128 up gives the same
+// factorization time within 0.3 ms.
+static int update_small_tiles() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("LAPACK_AMD_UPD64_TILES");
+    v = (e && *e) ? std::atoi(e) : 1024;
+    if (v < 0) v = 0;
+  }
+  return v;
+}
+
 template <class T>
 void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long ldb, T* C, long ldc, int nskip,
                      hipStream_t s) {
@@ -527,12 +546,12 @@ void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long
     return;
   }
   const int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
-  if (nskip == 0 && (m <= 64 || tm * tn < 128)) {
-    // narrow (one leaf's columns) or too few 128x128 tiles to occupy half
-    // the CUs (the next panel's columns late in the factorization): 64x64
-    // tiles spread the same work over four times the workgroups
+  if (nskip == 0 && (m <= 64 || tm * tn < update_small_tiles())) {
+    // narrow (one leaf's columns), or too small for update_big with the
+    // whole GPU to itself (the next panel's columns): 64x64 tiles spread the
+    // work over four times the workgroups of 128x128 tiles
     const int tm64 = (m + 63) / 64, tn64 = (n + 63) / 64;
-    gemm64_kernel<T><<<dim3(tm64 * tn64), 256, 0, s>>>(m, n, k, T(-1), A, lda, B, ldb, T(1), C, ldc, tm64);
+    gemm64_kernel<T, true><<<dim3(tm64 * tn64), 256, 0, s>>>(m, n, k, T(-1), A, lda, B, ldb, T(1), C, ldc, tm64);
     return;
   }
   const int chunk = (tm * tn + nx - 1) / nx;
