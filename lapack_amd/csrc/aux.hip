@@ -279,32 +279,35 @@ __global__ __launch_bounds__(256) void panel_boundary_kernel(T* __restrict__ Tm,
   }
   // 64 T rows at a time: every source value of the chunk in registers
   // (32 per thread), one barrier, every destination (chunks touch disjoint
-  // rows, so the next chunk's loads need no barrier after these stores)
+  // rows, so the next chunk's loads need no barrier after these stores).
+  // Thread tid moves T row r0 + tid % 64 of pairs tid / 64 + 4u, u < 32:
+  // the same pairs in every chunk, so their indices are read once.
   constexpr int CH = 64;
   constexpr int PER = 2 * W * CH / 256;
+  static_assert(256 % CH == 0 && PER * (256 / CH) == 2 * W, "pair lanes");
+  int psrc[PER], pdst[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int q = tid / CH + (256 / CH) * u;
+    psrc[u] = q < np ? s_out[2 * q + 1] : 0;
+    pdst[u] = q < np ? s_out[2 * q] : 0;
+  }
   for (int r0 = 0; r0 < tot; r0 += CH) {
+    const int ri = r0 + tid % CH;
+    const bool okr = ri < tot;
+    const long trow = ri < n1 ? (long)j + ri : (long)jj + w + (ri - n1);
     T v[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int e = tid + 256 * u;
-      const int rr = e % CH, q = e / CH;
-      const int ri = r0 + rr;
-      v[u] = T(0);
-      if (q < np && ri < tot) {
-        const long trow = ri < n1 ? (long)j + ri : (long)jj + w + (ri - n1);
-        v[u] = Tm[trow + (long)s_out[2 * q + 1] * ldt];
-      }
+      const int q = tid / CH + (256 / CH) * u;
+      v[u] = (okr && q < np) ? Tm[trow + (long)psrc[u] * ldt] : T(0);
     }
+    wait_vm();  // every source read before any destination is written
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int e = tid + 256 * u;
-      const int rr = e % CH, q = e / CH;
-      const int ri = r0 + rr;
-      if (q < np && ri < tot) {
-        const long trow = ri < n1 ? (long)j + ri : (long)jj + w + (ri - n1);
-        Tm[trow + (long)s_out[2 * q] * ldt] = v[u];
-      }
+      const int q = tid / CH + (256 / CH) * u;
+      if (okr && q < np) Tm[trow + (long)pdst[u] * ldt] = v[u];
     }
   }
   if (rest <= 0) return;
@@ -319,12 +322,22 @@ __global__ __launch_bounds__(256) void panel_boundary_kernel(T* __restrict__ Tm,
     T x[W];
 #pragma unroll
     for (int i = 0; i < W; ++i) x[i] = i < w ? Tm[trow + (long)(jj + i) * ldt] : T(0);
+    // row k+1 of the triangle is read into registers before row k is
+    // applied, so its LDS reads are in flight while row k's FMAs run (read
+    // inside the branch, only two reads were in flight at a time)
+    T lc[W], ln[W];
+#pragma unroll
+    for (int i = 1; i < W; ++i) lc[i] = sL[0][i];
 #pragma unroll
     for (int k = 0; k < W; ++k) {
+#pragma unroll
+      for (int i = k + 2; i < W; ++i) ln[i] = sL[k + 1][i];
       if (x[k] != T(0)) {
 #pragma unroll
-        for (int i = k + 1; i < W; ++i) x[i] = x[i] - x[k] * sL[k][i];
+        for (int i = k + 1; i < W; ++i) x[i] = x[i] - x[k] * lc[i];
       }
+#pragma unroll
+      for (int i = k + 2; i < W; ++i) lc[i] = ln[i];
     }
 #pragma unroll
     for (int i = 0; i < W; ++i)
