@@ -482,6 +482,109 @@ __global__ __launch_bounds__(256) void swaps_to_perm_blocked_kernel(int nrows, i
   }
 }
 
+// The same permutation in two launches, for long sequences: every block of
+// 256 interchanges is composed at once, one workgroup per block
+// (compose_blocks_kernel: block b's pairs at gp + b*PERM_PB, its pair count
+// -- or -1 for a block not of DGETRF's form -- at gp[b*PERM_PB +
+// 2*SWP_MAXPAIRS]); then one workgroup applies the blocks to the position
+// map in order (swaps_to_perm_pairs_kernel), reading block b+1's pairs
+// while it applies block b's, so a block costs an LDS gather and two
+// barriers instead of a composition.
+constexpr int PERM_PB = 2 * SWP_MAXPAIRS + 1;
+
+__global__ __launch_bounds__(256) void compose_blocks_kernel(int nrows, int k1, int k2,
+                                                             const int* __restrict__ ipiv, int* __restrict__ gp) {
+  __shared__ __align__(16) int s_p[256];
+  __shared__ int s_r[256];
+  __shared__ int s_out[2 * SWP_MAXPAIRS];
+  __shared__ int s_cnt;
+  const int b0 = k1 + 256 * (int)blockIdx.x;
+  if (b0 >= k2) return;
+  const int K = min(256, k2 - b0);
+  int* out = gp + (long)blockIdx.x * PERM_PB;
+  bool bad = false;
+  s_p[threadIdx.x] = -1;
+  if ((int)threadIdx.x < K) {
+    const int q = b0 + threadIdx.x;
+    int p = ipiv[q] - 1;
+    if (p < 0 || p >= nrows) p = q;  // ignored, as a self-interchange
+    if (p < q) bad = true;
+    s_p[threadIdx.x] = p;
+  }
+  if (threadIdx.x == 0) s_cnt = 0;
+  if (__syncthreads_or(bad ? 1 : 0) != 0) {
+    if (threadIdx.x == 0) out[2 * SWP_MAXPAIRS] = -1;
+    return;
+  }
+  compose_block(b0, K, s_p, s_r, s_out, &s_cnt);
+  const int np = s_cnt;
+  for (int o = threadIdx.x; o < 2 * np; o += 256) out[o] = s_out[o];
+  if (threadIdx.x == 0) out[2 * SWP_MAXPAIRS] = np;
+}
+
+__global__ __launch_bounds__(256) void swaps_to_perm_pairs_kernel(int nrows, int k1, int k2,
+                                                                  const int* __restrict__ ipiv, int incx,
+                                                                  const int* __restrict__ gp,
+                                                                  int* __restrict__ perm) {
+  extern __shared__ int cur[];  // nrows ints
+  const int tid = threadIdx.x;
+  for (int r = tid; r < nrows; r += 256) cur[r] = r;
+  const int nb = (k2 - k1 + 255) / 256;
+  // block b's pair slots o = tid and tid + 256 (at most 2 * 256 pairs)
+  int cnt = gp[2 * SWP_MAXPAIRS], d[2], sr[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    d[h] = gp[2 * (tid + 256 * h)];
+    sr[h] = gp[2 * (tid + 256 * h) + 1];
+  }
+  __syncthreads();
+  for (int b = 0; b < nb; ++b) {
+    // the next block's pairs, in flight while this block is applied
+    const int bn = min(b + 1, nb - 1);
+    const int* gn = gp + (long)bn * PERM_PB;
+    const int cntn = gn[2 * SWP_MAXPAIRS];
+    int dn[2], srn[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      dn[h] = gn[2 * (tid + 256 * h)];
+      srn[h] = gn[2 * (tid + 256 * h) + 1];
+    }
+    if (cnt >= 0) {
+      int v[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) v[h] = tid + 256 * h < cnt ? cur[sr[h]] : 0;
+      __syncthreads();
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (tid + 256 * h < cnt) cur[d[h]] = v[h];
+    } else if (tid == 0) {
+      const int b0 = k1 + 256 * b, K = min(256, k2 - b0);
+      for (int t = 0; t < K; ++t) {
+        const int i = b0 + t;
+        int p = ipiv[i] - 1;
+        if (p < 0 || p >= nrows) p = i;
+        if (p != i) {
+          const int a = cur[i];
+          cur[i] = cur[p];
+          cur[p] = a;
+        }
+      }
+    }
+    __syncthreads();
+    cnt = cntn;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      d[h] = dn[h];
+      sr[h] = srn[h];
+    }
+  }
+  if (incx > 0) {
+    for (int r = tid; r < nrows; r += 256) perm[r] = cur[r];
+  } else {
+    for (int r = tid; r < nrows; r += 256) perm[cur[r]] = r;
+  }
+}
+
 // dst(r,j) = src(perm[r], j) for r < nrows, j < ncols.
 template <class T>
 __global__ __launch_bounds__(256) void gather_rows_kernel(int nrows, int ncols, const int* __restrict__ perm,
@@ -669,9 +772,20 @@ void launch_apply_pairs_rows_s(float* B, long ldb, int ncols, const int* pairs, 
   if (ncols > 0) apply_pairs_rows_kernel<float><<<(ncols + SWP_ROWS - 1) / SWP_ROWS, 256, 0, s>>>(B, ldb, ncols, pairs, npairs);
 }
 
+size_t perm_pairs_ints(int k1, int k2) { return k2 > k1 ? (size_t)((k2 - k1 + 255) / 256) * PERM_PB : 0; }
+
 // perm (nrows ints) <- permutation of the interchange sequence; see above.
-void launch_swaps_to_perm(int nrows, int k1, int k2, const int* ipiv, int incx, int* perm, hipStream_t s) {
+// gp: perm_pairs_ints(k1, k2) ints of device scratch, or null (then the
+// blocks are composed one after another in one workgroup).
+void launch_swaps_to_perm(int nrows, int k1, int k2, const int* ipiv, int incx, int* perm, hipStream_t s,
+                          int* gp) {
   if (k1 >= 0 && k2 <= nrows && sizeof(int) * (size_t)nrows + 8192 <= (size_t)lds_limit()) {
+    if (gp && k2 > k1) {
+      const int nb = (k2 - k1 + 255) / 256;
+      compose_blocks_kernel<<<nb, 256, 0, s>>>(nrows, k1, k2, ipiv, gp);
+      swaps_to_perm_pairs_kernel<<<1, 256, sizeof(int) * (size_t)nrows, s>>>(nrows, k1, k2, ipiv, incx, gp, perm);
+      return;
+    }
     swaps_to_perm_blocked_kernel<<<1, 256, sizeof(int) * (size_t)nrows, s>>>(nrows, k1, k2, ipiv, incx, perm);
     return;
   }

@@ -83,7 +83,9 @@ void launch_apply_pairs_rows_s(float* B, long ldb, int ncols, const int* pairs, 
 // DLASWP on a column-major matrix: perm (nrows ints) <- the permutation of
 // the interchange sequence ipiv[k1..k2-1] (1-based entries, incx = +1/-1),
 // then rows are gathered through it.
-void launch_swaps_to_perm(int nrows, int k1, int k2, const int* ipiv, int incx, int* perm, hipStream_t s);
+void launch_swaps_to_perm(int nrows, int k1, int k2, const int* ipiv, int incx, int* perm, hipStream_t s,
+                          int* gp = nullptr);
+size_t perm_pairs_ints(int k1, int k2);  // device scratch of launch_swaps_to_perm's two-launch form
 void launch_gather_rows_d(int nrows, int ncols, const int* perm, const double* src, long lds, double* dst,
                           long ldd, hipStream_t s);
 void launch_gather_rows_s(int nrows, int ncols, const int* perm, const float* src, long lds, float* dst,

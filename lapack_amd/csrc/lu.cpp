@@ -610,6 +610,7 @@ size_t getrs_ws_bytes(int n, int nrhs, int nbs) {
   cv.take<T>((size_t)nbs * (size_t)((n + nbs - 1) / nbs) * nbs);
   cv.take<unsigned>(tri_chain_ctl_words(n));
   cv.take<int>(1);
+  cv.take<int>(perm_pairs_ints(0, n));
   return cv.off;
 }
 
@@ -648,6 +649,7 @@ int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* i
   T* WU = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nbs * nblk * nbs));
   unsigned* ctl = reinterpret_cast<unsigned*>(ws + cv.take<unsigned>(tri_chain_ctl_words(n)));
   int* cerr = reinterpret_cast<int*>(ws + cv.take<int>(1));
+  int* gpairs = reinterpret_cast<int*>(ws + cv.take<int>(perm_pairs_ints(0, n)));
   const long ldx = n, ldy = nbs, ldw = nbs;
   auto Ap = [&](long r, long col) { return A + r + col * lda; };
 
@@ -661,7 +663,7 @@ int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* i
   if (!getrs_use_gemm()) {
     launch_fill_i32(cerr, 1, 0, s);
     if (notran) {
-      launch_swaps_to_perm(n, 0, n, ipiv, 1, perm, s);
+      launch_swaps_to_perm(n, 0, n, ipiv, 1, perm, s, gpairs);
       K<T>::gather(n, nrhs, perm, B, ldb, X, ldx, s);                // X = P*B (dgetrs.f:187)
       launch_tri_chain<T>(0, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // L Y = X (dgetrs.f:191)
       launch_tri_chain<T>(1, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // U X = Y (dgetrs.f:197)
@@ -670,7 +672,7 @@ int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* i
       K<T>::copy(n, nrhs, B, ldb, X, ldx, s);
       launch_tri_chain<T>(2, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // U^T Y = B (dgetrs.f:205)
       launch_tri_chain<T>(3, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // L^T X = Y (dgetrs.f:211)
-      launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s);               // dgetrs.f:217
+      launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s, gpairs);               // dgetrs.f:217
       K<T>::gather(n, nrhs, perm, X, ldx, B, ldb, s);
     }
     if (pt.mark(-1)) fl.push_back(0);
@@ -681,7 +683,7 @@ int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* i
   K<T>::trtri_blocks(false, true, n, nbs, A, lda, WL, ldw, s);   // inv(L_dd), unit lower
   K<T>::trtri_blocks(true, false, n, nbs, A, lda, WU, ldw, s);   // inv(U_dd), non-unit upper
   if (notran) {
-    launch_swaps_to_perm(n, 0, n, ipiv, 1, perm, s);
+    launch_swaps_to_perm(n, 0, n, ipiv, 1, perm, s, gpairs);
     K<T>::gather(n, nrhs, perm, B, ldb, X, ldx, s);                // X = P*B (dgetrs.f:187)
     for (int d = 0; d < nblk; ++d) {                               // L Y = X (dgetrs.f:191)
       const int b0 = d * nbs, bs = std::min(nbs, n - b0);
@@ -716,7 +718,7 @@ int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* i
       if (b0 > 0)
         K<T>::gemm('T', 'N', b0, nrhs, bs, T(-1), Ap(b0, 0), lda, X + b0, ldx, T(1), X, ldx, s);
     }
-    launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s);              // dgetrs.f:217
+    launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s, gpairs);              // dgetrs.f:217
     K<T>::gather(n, nrhs, perm, X, ldx, B, ldb, s);
   }
   if (pt.mark(-1)) fl.push_back(0);
