@@ -60,6 +60,15 @@ __global__ __launch_bounds__(256) void tri_chain_kernel(int n, int nrhs, const T
   const int bi = LOWERM ? t : nblk - 1 - t;
   const int r0 = bi * TB;
   const int bs = min(TB, n - r0);
+  // this block's right-hand sides (rows only this workgroup writes), read
+  // now so that the chain step after the last dependency does not wait for them
+  T yb[QW];
+#pragma unroll
+  for (int u = 0; u < QW; ++u) {
+    const int q = QW * (tid >> 6) + u;
+    yb[u] = ((tid >> 6) < NWQ && (tid & (TB - 1)) < bs && q < nrhs) ? X[(long)(r0 + (tid & (TB - 1))) + (long)q * ldx]
+                                                                 : T(0);
+  }
 
   {
     T dv[TB * TB / 256];
@@ -210,7 +219,7 @@ __global__ __launch_bounds__(256) void tri_chain_kernel(int n, int nrhs, const T
     const int q = QW * g + u;
     T y = T(0);
     if (g < NWQ && lane < bs && q < nrhs) {
-      y = X[(long)(r0 + lane) + (long)q * ldx];
+      y = yb[u];
       y = y + (((red[(0 * TB + lane) * QN + q] + red[(1 * TB + lane) * QN + q]) + red[(2 * TB + lane) * QN + q]) +
                red[(3 * TB + lane) * QN + q]);
     }
