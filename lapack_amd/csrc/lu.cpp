@@ -508,7 +508,7 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
     // it runs multi-workgroup; the rest of the step leaves it.  A taller
     // panel (more groups) shares the GPU instead: measured at n = 32768,
     // reserving a second XCD slows the then dominant update more than it
-    // speeds the panel.
+    // speeds the panel (in FP32 as well: DSGESV 467 instead of 406 ms).
     const int pg = jbn > 0 ? panel_groups(m - jn) : 0;
     const int npx = (xcdsplit && pg > 0 && pg <= 32) ? 1 : 0;
     const bool split = npx > 0;
