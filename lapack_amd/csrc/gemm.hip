@@ -501,13 +501,14 @@ void launch_gemm_t(char ta, char tb, int m, int n, int k, T alpha, const T* A, l
     gemm_kernel<T, true, true, false><<<grid, NT, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, chunk, nx);
 }
 
-// CUs per XCD a persistent update may hold (LAPACK_AMD_UPD_CUS, default 28
-// of 32).
+// CUs per XCD a persistent update may hold (LAPACK_AMD_UPD_CUS, default 24
+// of 32: at n = 16384, 0.25-0.35 ms per factorization faster than 28; 22 and
+// 26 slower than 28, 16 by 5 ms, 32 by 9 ms).
 static int update_cus_per_xcd() {
   static int v = -1;
   if (v < 0) {
     const char* e = std::getenv("LAPACK_AMD_UPD_CUS");
-    v = (e && *e) ? std::atoi(e) : 28;
+    v = (e && *e) ? std::atoi(e) : 24;
     if (v < 1) v = 1;
   }
   return v;
