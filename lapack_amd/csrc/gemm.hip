@@ -514,6 +514,20 @@ static int update_cus_per_xcd() {
   return v;
 }
 
+// An update beside the panel below this many flops runs persistent on
+// update_cus_per_xcd() CUs per XCD (LAPACK_AMD_UPD_PERSIST_GF, in GFLOP;
+// default 60: at n = 16384, 30 is 0.5 ms per factorization slower, 90 and
+// 140 are 5 and 13 ms slower).
+static double update_persist_flops() {
+  static double v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("LAPACK_AMD_UPD_PERSIST_GF");
+    v = 1e9 * ((e && *e) ? std::atof(e) : 60.0);
+    if (v < 0) v = 0;
+  }
+  return v;
+}
+
 // Updates with fewer 128x128 tiles than this (and no XCD left to a panel)
 // take 64x64 tiles; at n = 16384 any threshold from 128 up gives the same
 // factorization time within 0.3 ms.
@@ -541,7 +555,7 @@ void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long
   if ((long)tmb * tnb >= 512) {
     const int chunk = (tmb * tnb + nx - 1) / nx;
     int per = chunk;
-    if (nskip > 0 && 2.0 * m * (double)n * k < 6e10) per = std::min(chunk, update_cus_per_xcd());
+    if (nskip > 0 && 2.0 * m * (double)n * k < update_persist_flops()) per = std::min(chunk, update_cus_per_xcd());
     update_big_kernel<T, 4, 2><<<dim3(per * NXCD), 512, 0, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tmb, tnb, chunk,
                                                                nx);
     return;
