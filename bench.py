@@ -43,6 +43,8 @@ def parse():
                    help="time the steps without per-phase events (no roofline / phase fields)")
     p.add_argument("--mp-half", action="store_true",
                    help="add the FP16 / BF16 GMRES-IR variants to the mixed-precision comparison")
+    p.add_argument("--extras", action="store_true",
+                   help="also time DGEEQU, DGEEQUB, DLAQGE, DGERFS and the adjoint at the benchmark's order")
     p.add_argument("--calib-copy-mib", type=int, default=0,
                    help="run one plain device copy of this many MiB first (calibrates HBM byte counters)")
     return p.parse_args()
@@ -86,6 +88,64 @@ def update_traffic(n):
         if line.startswith("traffic_bytes_per_launch:"):
             return int(line.split(":")[1])
     return None
+
+
+def extras(n, nrhs, seed):
+    """Times (ms, device-resident operands, synchronous entry points, mean of
+    three calls after one untimed call) of the routines around the solve at
+    order n: DGEEQU and DGEEQUB on A, DLAQGE with their factors, DGERFS('N')
+    refining the DGETRS solution of nrhs right-hand sides, and the adjoint
+    (Bbar = A^-T Xbar, Abar = -Bbar X^T, an n x n output) on the saved LU."""
+    import numpy as np
+    import torch
+    import lapack_amd as la
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    a = torch.randn(n, n, generator=g, device=dev, dtype=torch.float64)  # column-major: A(i, j) = a[j, i]
+    b = torch.randn(nrhs, n, generator=g, device=dev, dtype=torch.float64)
+    r = torch.empty(n, device=dev, dtype=torch.float64)
+    c = torch.empty(n, device=dev, dtype=torch.float64)
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return round((time.perf_counter() - t0) / reps * 1e3, 3)
+
+    out = {"n": n, "nrhs": nrhs}
+    out["dgeequ_ms"] = timed(lambda: la.dgeequ(n, n, a, n, r, c))
+    out["dgeequb_ms"] = timed(lambda: la.dgeequ(n, n, a, n, r, c, True))
+    info, rowcnd, colcnd, amax = la.dgeequ(n, n, a, n, r, c)
+    assert info == 0
+    a2 = a.clone()
+    out["dlaqge_ms"] = timed(lambda: la.dlaqge(n, n, a2, n, r, c, rowcnd, colcnd, amax))
+    del a2
+    lu = a.clone()
+    ipiv = torch.empty(n, device=dev, dtype=torch.int32)
+    assert la.dgetrf(n, n, lu, n, ipiv) == 0
+    x = b.clone()
+    assert la.dgetrs("N", n, nrhs, lu, n, ipiv, x, n) == 0
+    x0 = x.clone()
+    ferr = np.zeros(nrhs)
+    berr = np.zeros(nrhs)
+
+    def rfs():
+        x.copy_(x0)
+        assert la.dgerfs("N", n, nrhs, a, n, lu, n, ipiv, b, n, x, n, ferr, berr) == 0
+    out["dgerfs_ms"] = timed(rfs)
+    out["dgerfs_berr_max"] = float(berr.max())
+    xbar = torch.randn(nrhs, n, generator=g, device=dev, dtype=torch.float64)
+    bbar = torch.empty(nrhs, n, device=dev, dtype=torch.float64)
+    abar = torch.empty(n, n, device=dev, dtype=torch.float64)
+
+    def adj():
+        assert la.dgesv_adjoint_async(n, nrhs, lu, n, ipiv, x, n, xbar, n, bbar, n, abar, n) == 0
+    out["adjoint_ms"] = timed(adj)
+    return out
 
 
 def mp_compare(n, seed, half=False):
@@ -295,6 +355,10 @@ def main():
         del d_as, d_bs, d_a0
         torch.cuda.empty_cache()
         out["mp_ir"] = mp_compare(args.mp_n, 32768, args.mp_half)
+    if args.extras:
+        torch.cuda.empty_cache()
+        print(f"[bench] DGEEQU/DGEEQUB/DLAQGE/DGERFS/adjoint at n={n}", file=sys.stderr, flush=True)
+        out["extras"] = extras(n, nrhs, args.seed)
     if not args.no_cpu:
         print(f"[bench] CPU baseline sample n={args.cpu_n}", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.seed)
