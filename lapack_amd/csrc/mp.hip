@@ -72,23 +72,46 @@ __global__ __launch_bounds__(256) void col_absmax_kernel(int m, int n, const dou
   if (threadIdx.x == 0) atomicMax(&out[j], s[0]);
 }
 
+// The row reductions run one thread per row in j order (the reference's
+// order), 64-thread workgroups so that every CU takes rows, and RU columns
+// loaded at a time so that RU loads per thread are in flight.
+constexpr int RU = 16;
+
 // rowsum[i] = sum_j |A(i,j)| (the WORK vector of DLANGE 'I'), j-order per row.
-__global__ __launch_bounds__(256) void row_abssum_kernel(int m, int n, const double* __restrict__ A, long lda,
-                                                         double* __restrict__ rowsum) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(64) void row_abssum_kernel(int m, int n, const double* __restrict__ A, long lda,
+                                                        double* __restrict__ rowsum) {
+  const long i = (long)blockIdx.x * 64 + threadIdx.x;
   if (i >= m) return;
+  const double* p = A + i;
   double s = 0.0;
-  for (int j = 0; j < n; ++j) s = s + fabs(A[i + (long)j * lda]);
+  int j = 0;
+  for (; j + RU <= n; j += RU) {
+    double v[RU];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) v[u] = p[(long)(j + u) * lda];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) s = s + fabs(v[u]);
+  }
+  for (; j < n; ++j) s = s + fabs(p[(long)j * lda]);
   rowsum[i] = s;
 }
 
 // Row maxima r(i) = max_j |A(i,j)| (DGEEQU(B) first pass).
-__global__ __launch_bounds__(256) void row_absmax_kernel(int m, int n, const double* __restrict__ A, long lda,
-                                                         double* __restrict__ r) {
-  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(64) void row_absmax_kernel(int m, int n, const double* __restrict__ A, long lda,
+                                                        double* __restrict__ r) {
+  const long i = (long)blockIdx.x * 64 + threadIdx.x;
   if (i >= m) return;
+  const double* p = A + i;
   double mx = 0.0;
-  for (int j = 0; j < n; ++j) mx = fmax(mx, fabs(A[i + (long)j * lda]));
+  int j = 0;
+  for (; j + RU <= n; j += RU) {
+    double v[RU];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) v[u] = p[(long)(j + u) * lda];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) mx = fmax(mx, fabs(v[u]));
+  }
+  for (; j < n; ++j) mx = fmax(mx, fabs(p[(long)j * lda]));
   r[i] = mx;
 }
 
@@ -96,8 +119,21 @@ __global__ __launch_bounds__(256) void row_absmax_kernel(int m, int n, const dou
 __global__ __launch_bounds__(256) void col_absmax_scaled_kernel(int m, int n, const double* __restrict__ A, long lda,
                                                                 const double* __restrict__ r, double* __restrict__ c) {
   const int j = blockIdx.x;
+  // eight rows per thread in flight at a time (a maximum is exact in any
+  // order, and fmax drops a NaN whichever order it meets it in)
   double mx = 0.0;
-  for (long i = threadIdx.x; i < m; i += 256) mx = fmax(mx, fabs(A[i + (long)j * lda]) * r[i]);
+  for (long i0 = threadIdx.x; i0 < m; i0 += 256 * 8) {
+    double v[8], w[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const long i = i0 + 256 * u;
+      v[u] = i < m ? A[i + (long)j * lda] : 0.0;
+      w[u] = i < m ? r[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (i0 + 256 * u < m) mx = fmax(mx, fabs(v[u]) * w[u]);
+  }
   __shared__ double s[256];
   s[threadIdx.x] = mx;
   __syncthreads();
@@ -194,10 +230,10 @@ void launch_col_absmax(int m, int n, const double* A, long lda, unsigned long lo
   col_absmax_kernel<<<dim3(gx, n), 256, 0, s>>>(m, n, A, lda, out);
 }
 void launch_row_abssum(int m, int n, const double* A, long lda, double* rowsum, hipStream_t s) {
-  if (m > 0) row_abssum_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, n, A, lda, rowsum);
+  if (m > 0) row_abssum_kernel<<<(m + 63) / 64, 64, 0, s>>>(m, n, A, lda, rowsum);
 }
 void launch_row_absmax(int m, int n, const double* A, long lda, double* r, hipStream_t s) {
-  if (m > 0) row_absmax_kernel<<<(m + 255) / 256, 256, 0, s>>>(m, n, A, lda, r);
+  if (m > 0) row_absmax_kernel<<<(m + 63) / 64, 64, 0, s>>>(m, n, A, lda, r);
 }
 void launch_col_absmax_scaled(int m, int n, const double* A, long lda, const double* r, double* c, hipStream_t s) {
   if (n > 0) col_absmax_scaled_kernel<<<n, 256, 0, s>>>(m, n, A, lda, r, c);
