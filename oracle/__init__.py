@@ -62,6 +62,8 @@ def o():
             "oracle_dget02": (d, [ch, i, i, i, vp, i, vp, i, vp, i]),
             "oracle_dget08": (d, [ch, i, i, i, vp, i, vp, i, vp, i]),
             "oracle_dgerfs": (i, [ch, i, i, vp, i, vp, i, vp, vp, i, vp, i, vp, vp]),
+            "oracle_dget04": (d, [i, i, vp, i, vp, i, d]),
+            "oracle_dget07": (None, [ch, i, i, vp, i, vp, i, vp, i, vp, i, vp, i, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -245,6 +247,29 @@ def dget08(a, x, b, trans="N"):
                              _p(b), max(1, b.shape[0]))
 
 
+def dget04(x, xact, rcond):
+    """DGET04 (TESTING/LIN/dget04.f): forward error times RCOND over eps."""
+    x = np.asfortranarray(x, dtype=np.float64).reshape(len(x), -1, order="F")
+    xact = np.asfortranarray(xact, dtype=np.float64).reshape(len(xact), -1, order="F")
+    n, nrhs = x.shape
+    return o().oracle_dget04(n, nrhs, _p(x), max(1, n), _p(xact), max(1, n), float(rcond))
+
+
+def dget07(a, b, x, xact, ferr, berr, trans="N", chkferr=True):
+    """DGET07 (TESTING/LIN/dget07.f): (FERR ratio, BERR ratio)."""
+    a = np.asfortranarray(a, dtype=np.float64)
+    n = a.shape[0]
+    b = np.asfortranarray(b, dtype=np.float64).reshape(n, -1, order="F")
+    x = np.asfortranarray(x, dtype=np.float64).reshape(n, -1, order="F")
+    xact = np.asfortranarray(xact, dtype=np.float64).reshape(n, -1, order="F")
+    ferr = np.ascontiguousarray(ferr, dtype=np.float64)
+    berr = np.ascontiguousarray(berr, dtype=np.float64)
+    out = np.zeros(2)
+    o().oracle_dget07(trans.encode(), n, x.shape[1], _p(a), max(1, n), _p(b), max(1, n), _p(x), max(1, n), _p(xact),
+                      max(1, n), _p(ferr), 1 if chkferr else 0, _p(berr), _p(out))
+    return float(out[0]), float(out[1])
+
+
 def dgerfs(a, lu, ipiv, b, x, trans="N"):
     """(X refined, FERR, BERR, INFO) of the reference DGERFS on factors lu/ipiv."""
     a = np.asfortranarray(a, dtype=np.float64)
@@ -274,6 +299,33 @@ def ref_dgetrf(a):
     r().dgetrf_(_ri(m), _ri(n), ctypes.c_void_p(_p(lu)), _ri(max(1, m)), ctypes.c_void_p(_p(ipiv)),
                 ctypes.byref(info))
     return lu, ipiv[:min(m, n)], info.value
+
+
+def ref_dget04(x, xact, rcond):
+    x = np.asfortranarray(x, dtype=np.float64).reshape(len(x), -1, order="F")
+    xact = np.asfortranarray(xact, dtype=np.float64).reshape(len(xact), -1, order="F")
+    n, nrhs = x.shape
+    resid = ctypes.c_double(0)
+    r().dget04_(_ri(n), _ri(nrhs), ctypes.c_void_p(_p(x)), _ri(max(1, n)), ctypes.c_void_p(_p(xact)),
+                _ri(max(1, n)), ctypes.byref(ctypes.c_double(rcond)), ctypes.byref(resid))
+    return resid.value
+
+
+def ref_dget07(a, b, x, xact, ferr, berr, trans="N", chkferr=True):
+    a = np.asfortranarray(a, dtype=np.float64)
+    n = a.shape[0]
+    b = np.asfortranarray(b, dtype=np.float64).reshape(n, -1, order="F")
+    x = np.asfortranarray(x, dtype=np.float64).reshape(n, -1, order="F")
+    xact = np.asfortranarray(xact, dtype=np.float64).reshape(n, -1, order="F")
+    ferr = np.ascontiguousarray(ferr, dtype=np.float64)
+    berr = np.ascontiguousarray(berr, dtype=np.float64)
+    out = np.zeros(2)
+    chk = ctypes.c_int(1 if chkferr else 0)  # Fortran LOGICAL, 4 bytes: .TRUE. = 1 for flang
+    r().dget07_(ctypes.c_char_p(trans.encode()), _ri(n), _ri(x.shape[1]), ctypes.c_void_p(_p(a)), _ri(max(1, n)),
+                ctypes.c_void_p(_p(b)), _ri(max(1, n)), ctypes.c_void_p(_p(x)), _ri(max(1, n)),
+                ctypes.c_void_p(_p(xact)), _ri(max(1, n)), ctypes.c_void_p(_p(ferr)), ctypes.byref(chk),
+                ctypes.c_void_p(_p(berr)), ctypes.c_void_p(_p(out)), ctypes.c_size_t(1))
+    return float(out[0]), float(out[1])
 
 
 def ref_sgetrf(a):

@@ -26,6 +26,9 @@
 //     closed form (TESTING/LIN/dchkeq.f:107-173) and equality with _ref.
 //   gemv, asum, lacn2, gerfs                                   -- bitwise
 //     equality with _ref's DGERFS (X, FERR, BERR) and DGET07's bounds.
+//   get04 / get07                                              -- equality
+//     with the reference's TESTING/LIN/dget04.f and dget07.f built into _ref,
+//     and closed forms (known errors, FERR too small, BERR at its bound).
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -802,6 +805,78 @@ double oracle_dget08(char trans, int m, int n, int nrhs, const double* a, int ld
     else resid = std::max(resid, ((bnorm / anorm) / xnorm) / eps);
   }
   return resid;
+}
+
+// DGET04: max_j ||x_j - xact_j||_inf / ||xact_j||_inf * RCOND / eps, the
+// forward error scaled by the condition estimate (TESTING/LIN/dget04.f:
+// 126-175): 1/eps when RCOND < 0 or an exact solution of zero is missed.
+double oracle_dget04(int n, int nrhs, const double* x, int ldx, const double* xact, int ldxact, double rcond) {
+  if (n <= 0 || nrhs <= 0) return 0.0;
+  const double eps = Mach<double>::eps;
+  if (rcond < 0) return 1.0 / eps;
+  double resid = 0.0;
+  for (int j = 1; j <= nrhs; ++j) {
+    const int ix = iamax(n, &AT(xact, 1, j, ldxact), 1);
+    const double xnorm = std::fabs(AT(xact, ix, j, ldxact));
+    double diffnm = 0.0;
+    for (int i = 1; i <= n; ++i) diffnm = std::max(diffnm, std::fabs(AT(x, i, j, ldx) - AT(xact, i, j, ldxact)));
+    if (xnorm <= 0) {
+      if (diffnm > 0) resid = 1.0 / eps;
+    } else {
+      resid = std::max(resid, (diffnm / xnorm) * rcond);
+    }
+  }
+  if (resid * eps < 1.0) resid = resid / eps;
+  return resid;
+}
+
+// DGET07: the two error-bound ratios of DGERFS (TESTING/LIN/dget07.f:
+// 190-281): reslts[0] = max_j (||x_j - xact_j||_inf / ||x_j||_inf) / FERR_j
+// (1/eps where FERR does not bound the error; only when chkferr), and
+// reslts[1] = max_j BERR_j / ((n+1) eps + (n+1) unfl / max(min_i (|b| +
+// |op(A)||x|)_i, (n+1) unfl)).
+void oracle_dget07(char trans, int n, int nrhs, const double* a, int lda, const double* b, int ldb,
+                   const double* x, int ldx, const double* xact, int ldxact, const double* ferr, int chkferr,
+                   const double* berr, double* reslts) {
+  if (n <= 0 || nrhs <= 0) {
+    reslts[0] = 0.0;
+    reslts[1] = 0.0;
+    return;
+  }
+  const double eps = Mach<double>::eps;
+  const double unfl = Mach<double>::sfmin;
+  const double ovfl = 1.0 / unfl;
+  const bool notran = (trans == 'N' || trans == 'n');
+  double errbnd = 0.0;
+  if (chkferr) {
+    for (int j = 1; j <= nrhs; ++j) {
+      const int imax = iamax(n, &AT(x, 1, j, ldx), 1);
+      const double xnorm = std::max(std::fabs(AT(x, imax, j, ldx)), unfl);
+      double diff = 0.0;
+      for (int i = 1; i <= n; ++i) diff = std::max(diff, std::fabs(AT(x, i, j, ldx) - AT(xact, i, j, ldxact)));
+      if (!(xnorm > 1.0) && !(diff <= ovfl * xnorm)) {
+        errbnd = 1.0 / eps;
+        continue;
+      }
+      if (diff / xnorm <= ferr[j - 1]) errbnd = std::max(errbnd, (diff / xnorm) / ferr[j - 1]);
+      else errbnd = 1.0 / eps;
+    }
+  }
+  reslts[0] = errbnd;
+  for (int k = 1; k <= nrhs; ++k) {
+    double axbi = 0.0;
+    for (int i = 1; i <= n; ++i) {
+      double tmp = std::fabs(AT(b, i, k, ldb));
+      if (notran) {
+        for (int j = 1; j <= n; ++j) tmp = tmp + std::fabs(AT(a, i, j, lda)) * std::fabs(AT(x, j, k, ldx));
+      } else {
+        for (int j = 1; j <= n; ++j) tmp = tmp + std::fabs(AT(a, j, i, lda)) * std::fabs(AT(x, j, k, ldx));
+      }
+      axbi = (i == 1) ? tmp : std::min(axbi, tmp);
+    }
+    const double tmp = berr[k - 1] / ((n + 1) * eps + (n + 1) * unfl / std::max(axbi, (n + 1) * unfl));
+    reslts[1] = (k == 1) ? tmp : std::max(reslts[1], tmp);
+  }
 }
 
 }  // extern "C"

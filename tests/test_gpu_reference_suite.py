@@ -1,0 +1,51 @@
+"""The reference's own GE-path test suite run through the library's drop-in
+symbols: oracle/_ref/lin/xlinge (built by oracle/Makefile from TESTING/LIN,
+TESTING/MATGEN, SRC, BLAS/SRC and INSTALL, linked against
+lapack_amd/liblapack_amd.so so that DGETRF, DGETRF2, DGETRS, DGESV, DLASWP,
+DSGESV, SGETRF, SGETRS, DGERFS, DGEEQU, DLAQGE and DGEMM -- every caller's,
+the test routines' included -- are the GPU library's, and XERBLA is the test
+suite's own TESTING/LIN/xerbla.f) with the parameter sets of TESTING/dtest.in
+(GE path) and TESTING/dstest.in (DSGESV):
+
+  * DCHKGE: DGETRF / DGETRS / DGERFS / DGETRI / DGECON over the 11 matrix
+    types of TESTING/LIN/dlatb4.f, test ratios DGET01..DGET07 below 30;
+  * DDRVGE: DGESV and DGESVX (equilibration, factor, RCOND, refinement);
+  * DERRGE / DERRVX / DERRAB: the error exits (parameter numbers, XERBLA);
+  * DDRVAB: DSGESV, DGET08 below sqrt(N) when ITER >= 0.
+
+The suite's own summary lines decide: every ALASUM / ALASVM line says "All
+tests ... passed", every error-exit line "passed", and no "***" message.
+"""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "oracle", "_ref", "lin", "xlinge")
+
+
+def test_reference_ge_suite_through_drop_in(dev):
+    if not os.path.exists(EXE):
+        pytest.skip("reference test suite not built (needs the reference sources at build time)")
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=600, cwd=os.path.dirname(EXE))
+    out = r.stdout
+    log = os.path.join(ROOT, "gpurun_out")
+    if os.path.isdir(log):
+        with open(os.path.join(log, "xlinge.out"), "w") as f:
+            f.write(out + r.stderr)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "END OF GE TESTS" in out
+    lines = out.splitlines()
+    bad = [l for l in lines if "***" in l or "failed" in l]
+    assert not bad, "\n".join(bad[:40])
+    passed = [l.strip() for l in lines if "passed" in l]
+    # DCHKGE + DERRGE, DDRVGE + DERRVX, DDRVAB + DERRAB
+    assert any("All tests for DGE routines passed" in l for l in passed), passed
+    assert sum("All tests for DGE drivers  passed" in l for l in passed) == 2, passed
+    assert any("DGE routines passed the tests of the error exits" in l for l in passed), passed
+    assert any("DGE drivers passed the tests of the error exits" in l for l in passed), passed
+    assert any("DSGESV drivers passed the tests of the error exits" in l for l in passed), passed
+    print("\n".join(passed))

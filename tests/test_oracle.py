@@ -382,3 +382,49 @@ def test_dgerfs_bounds():
     # the bound of the computed problem holds up to the O(eps) perturbation b carries
     assert np.all(err <= ferr + 30 * eps * np.linalg.cond(a, np.inf))
     assert np.all(ferr < 1e-10)
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_dget04_dget07_vs_reference(trans):
+    """DGET04 / DGET07 restated (TESTING/LIN/dget04.f, dget07.f) equal the
+    reference's own routines built into oracle/_ref, bit for bit."""
+    if not oracle.ref_available():
+        pytest.skip("reference build absent")
+    rng = np.random.default_rng(7)
+    for n, nrhs in [(1, 1), (9, 2), (60, 3)]:
+        a = np.asfortranarray(rng.uniform(-1, 1, (n, n)))
+        xact = np.asfortranarray(rng.uniform(-1, 1, (n, nrhs)))
+        x = np.asfortranarray(xact + rng.uniform(-1, 1, (n, nrhs)) * 1e-11)
+        b = np.asfortranarray((a.T if trans == "T" else a) @ xact)
+        ferr = rng.uniform(1e-12, 1e-9, nrhs)
+        berr = rng.uniform(0, 1e-15, nrhs)
+        for chk in (True, False):
+            assert oracle.dget07(a, b, x, xact, ferr, berr, trans, chk) == \
+                oracle.ref_dget07(a, b, x, xact, ferr, berr, trans, chk)
+        for rcond in (0.25, 1e-7, -1.0):
+            assert oracle.dget04(x, xact, rcond) == oracle.ref_dget04(x, xact, rcond)
+
+
+def test_dget04_dget07_closed_forms():
+    eps = 2.0 ** -53
+    n = 8
+    xact = np.ones((n, 1), order="F") * 2.0
+    x = xact.copy(order="F")
+    x[3, 0] += 2.0 ** -20
+    # DGET04: (max|x - xact| / max|xact|) * RCOND / eps
+    assert oracle.dget04(x, xact, 0.5) == (2.0 ** -20 / 2.0) * 0.5 / eps
+    assert oracle.dget04(x, xact, -1.0) == 1.0 / eps
+    assert oracle.dget04(x, np.zeros((n, 1), order="F"), 0.5) == 1.0 / eps
+    # a planted mistake -- the error taken relative to x instead of xact -- changes it
+    assert oracle.dget04(x, xact, 0.5) != (2.0 ** -20 / np.abs(x).max()) * 0.5 / eps
+    # DGET07 on A = I: |b| + |A||x| = 4 everywhere, so BERR = (n+1) eps gives ratio 1
+    a = np.eye(n, order="F")
+    b = xact.copy(order="F")
+    r0, r1 = oracle.dget07(a, b, x, xact, np.array([1e-3]), np.array([(n + 1) * eps]))
+    assert r1 == pytest.approx(1.0, rel=1e-15)
+    assert r0 == pytest.approx((2.0 ** -20 / (2.0 + 2.0 ** -20)) / 1e-3, rel=1e-15)
+    # FERR smaller than the true error: 1/eps
+    r0, _ = oracle.dget07(a, b, x, xact, np.array([1e-9]), np.array([0.0]))
+    assert r0 == 1.0 / eps
+    # without the FERR check the first ratio is 0
+    assert oracle.dget07(a, b, x, xact, np.array([1e-9]), np.array([0.0]), chkferr=False)[0] == 0.0
