@@ -34,7 +34,7 @@ extern "C" {
 /* ---- library error codes (INFO values below every LAPACK argument code) -- */
 #define LA_ERR_HIP (-1001)      /* a HIP runtime call failed                   */
 #define LA_ERR_NOMEM (-1002)    /* device allocation failed                    */
-#define LA_ERR_SYNC (-1003)     /* a cooperative panel kernel's barrier timed out */
+#define LA_ERR_SYNC (-1003)     /* a bounded wait between workgroups gave up   */
 #define LA_ERR_NODEVICE (-1004) /* no GPU visible                              */
 #define LA_ERR_LIMIT (-1005)    /* a size beyond what this build supports      */
 
@@ -94,6 +94,31 @@ void dgeequb_(const int* m, const int* n, const double* a, const int* lda, doubl
 void dlaqge_(const int* m, const int* n, double* a, const int* lda, const double* r, const double* c,
              const double* rowcnd, const double* colcnd, const double* amax, char* equed, size_t equed_len);
 
+/* DLANGE: 'M' max |a|, '1'/'O' one norm, 'I' infinity norm, 'F'/'E'
+ * Frobenius norm of a host or device matrix, computed on the GPU (sums in
+ * the reference's order).  Replaces SRC/dlange.f:114. */
+double dlange_(const char* norm, const int* m, const int* n, const double* a, const int* lda, double* work,
+               size_t norm_len);
+
+/* DGECON: reciprocal condition number estimate from the factors of DGETRF
+ * (DLACN2 on the host, the triangular solves on the GPU).  Replaces
+ * SRC/dgecon.f:130 (LAPACKE/include/lapack.h, dgecon_). */
+void dgecon_(const char* norm, const int* n, const double* a, const int* lda, const double* anorm, double* rcond,
+             double* work, int* iwork, int* info, size_t norm_len);
+
+/* DGETRI: inverse from the factors of DGETRF.  Replaces SRC/dgetri.f:114
+ * (LAPACKE/include/lapack.h, dgetri_); WORK(1) returns N*64 as the
+ * reference's optimal size, LWORK = -1 is a size query. */
+void dgetri_(const int* n, double* a, const int* lda, const int* ipiv, double* work, const int* lwork, int* info);
+
+/* DGESVX: the expert driver -- equilibration, LU, RCOND, solve, refinement
+ * with error bounds.  Replaces SRC/dgesvx.f:349 (LAPACKE/include/lapack.h,
+ * dgesvx_).  WORK(1) returns the reciprocal pivot growth. */
+void dgesvx_(const char* fact, const char* trans, const int* n, const int* nrhs, double* a, const int* lda, double* af,
+             const int* ldaf, int* ipiv, char* equed, double* r, double* c, double* b, const int* ldb, double* x,
+             const int* ldx, double* rcond, double* ferr, double* berr, double* work, int* iwork, int* info,
+             size_t fact_len, size_t trans_len, size_t equed_len);
+
 /* DGEMM: C <- alpha*op(A)*op(B) + beta*C on the FP64 matrix cores.
  * Replaces BLAS/SRC/dgemm.f:213. */
 void dgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k,
@@ -140,6 +165,13 @@ int lapack_amd_dgerfs(char trans, int n, int nrhs, const double* a, int lda, con
                       const int* ipiv, const double* b, int ldb, double* x, int ldx, double* ferr, double* berr);
 int lapack_amd_dgeequ(int m, int n, const double* a, int lda, double* r, double* c, double* rowcnd, double* colcnd,
                       double* amax);
+double lapack_amd_dlange(char norm, int m, int n, const double* a, int lda);
+int lapack_amd_dgecon(char norm, int n, const double* a, int lda, double anorm, double* rcond);
+int lapack_amd_dgetri(int n, double* a, int lda, const int* ipiv);
+/* DGESVX semantics; *rpvgrw receives WORK(1), the reciprocal pivot growth. */
+int lapack_amd_dgesvx(char fact, char trans, int n, int nrhs, double* a, int lda, double* af, int ldaf, int* ipiv,
+                      char* equed, double* r, double* c, double* b, int ldb, double* x, int ldx, double* rcond,
+                      double* ferr, double* berr, double* rpvgrw);
 int lapack_amd_dgeequb(int m, int n, const double* a, int lda, double* r, double* c, double* rowcnd,
                        double* colcnd, double* amax);
 char lapack_amd_dlaqge(int m, int n, double* a, int lda, const double* r, const double* c, double rowcnd,
@@ -205,6 +237,57 @@ int lapack_amd_sgetrs_batched_async(const lapack_amd_batch_plan* plan, const int
                                     const int* d_lda, int* const* d_ipiv, float* const* d_b, const int* d_ldb,
                                     void* stream);
 
+/* ====================== 5. LAPACKE C interface ============================ */
+
+/* The LAPACKE functions of the path (LAPACKE/include/lapacke.h; lapack_int
+ * is int, the LP64 build): the layout check (-1), the NaN scan of the
+ * inputs (LAPACKE_set_nancheck / the LAPACKE_NANCHECK environment variable),
+ * INFO of the Fortran routine shifted by one when negative, and for
+ * LAPACK_ROW_MAJOR the leading-dimension checks and column-major
+ * temporaries -- as LAPACKE/src/lapacke_dgetrf.c, lapacke_dgetrf_work.c and
+ * their siblings.  Host or device arrays (row-major device arrays are
+ * transposed on the GPU). */
+#ifndef LAPACK_ROW_MAJOR
+#define LAPACK_ROW_MAJOR 101
+#define LAPACK_COL_MAJOR 102
+#endif
+#ifndef LAPACK_WORK_MEMORY_ERROR
+#define LAPACK_WORK_MEMORY_ERROR (-1010)
+#define LAPACK_TRANSPOSE_MEMORY_ERROR (-1011)
+#endif
+void LAPACKE_xerbla(const char* name, int info);
+void LAPACKE_set_nancheck(int flag);
+int LAPACKE_get_nancheck(void);
+int LAPACKE_dgetrf(int matrix_layout, int m, int n, double* a, int lda, int* ipiv);
+int LAPACKE_dgetrf_work(int matrix_layout, int m, int n, double* a, int lda, int* ipiv);
+int LAPACKE_dgetrf2(int matrix_layout, int m, int n, double* a, int lda, int* ipiv);
+int LAPACKE_dgetrf2_work(int matrix_layout, int m, int n, double* a, int lda, int* ipiv);
+int LAPACKE_dgetrs(int matrix_layout, char trans, int n, int nrhs, const double* a, int lda, const int* ipiv,
+                   double* b, int ldb);
+int LAPACKE_dgetrs_work(int matrix_layout, char trans, int n, int nrhs, const double* a, int lda, const int* ipiv,
+                        double* b, int ldb);
+int LAPACKE_dgesv(int matrix_layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb);
+int LAPACKE_dgesv_work(int matrix_layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb);
+int LAPACKE_dgetri(int matrix_layout, int n, double* a, int lda, const int* ipiv);
+int LAPACKE_dgetri_work(int matrix_layout, int n, double* a, int lda, const int* ipiv, double* work, int lwork);
+int LAPACKE_dsgesv(int matrix_layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb, double* x,
+                   int ldx, int* iter);
+int LAPACKE_dsgesv_work(int matrix_layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb,
+                        double* x, int ldx, double* work, float* swork, int* iter);
+int LAPACKE_dgecon(int matrix_layout, char norm, int n, const double* a, int lda, double anorm, double* rcond);
+int LAPACKE_dgecon_work(int matrix_layout, char norm, int n, const double* a, int lda, double anorm, double* rcond,
+                        double* work, int* iwork);
+double LAPACKE_dlange(int matrix_layout, char norm, int m, int n, const double* a, int lda);
+double LAPACKE_dlange_work(int matrix_layout, char norm, int m, int n, const double* a, int lda, double* work);
+int LAPACKE_dlaswp(int matrix_layout, int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx);
+int LAPACKE_dlaswp_work(int matrix_layout, int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx);
+int LAPACKE_dgesvx(int matrix_layout, char fact, char trans, int n, int nrhs, double* a, int lda, double* af,
+                   int ldaf, int* ipiv, char* equed, double* r, double* c, double* b, int ldb, double* x, int ldx,
+                   double* rcond, double* ferr, double* berr, double* rpivot);
+int LAPACKE_dgesvx_work(int matrix_layout, char fact, char trans, int n, int nrhs, double* a, int lda, double* af,
+                        int ldaf, int* ipiv, char* equed, double* r, double* c, double* b, int ldb, double* x,
+                        int ldx, double* rcond, double* ferr, double* berr, double* work, int* iwork);
+
 /* ============================ tuning and tools ============================ */
 
 /* Block size NB of the outer loop (the ILAENV(1,'DGETRF') value, reference
@@ -213,6 +296,25 @@ int lapack_amd_sgetrs_batched_async(const lapack_amd_batch_plan* plan, const int
  * suite's XLAENV hook (TESTING/LIN/xlaenv.f:77). */
 void lapack_amd_set_tuning(int nb, int leaf);
 void lapack_amd_get_tuning(int* nb, int* leaf);
+
+/* Autotuner of the plain factorization (the role ILAENV's NB,
+ * SRC/ilaenv.f:289-295, plays in the reference): times DGETRF of an n x n
+ * matrix on the current device for each candidate NB in {256, 192, 128} and
+ * leaf width in {64, 32}, keeps the fastest for n's bucket (the next power
+ * of two >= n), and with a non-null path writes the table there as JSON
+ * {"getrf": {"<bucket>": {"nb": .., "leaf": .., "ms": ..}}} (merged with the
+ * buckets the file already holds).  DGETRF then uses the table's entry for
+ * min(m, n)'s bucket unless lapack_amd_set_tuning or LAPACK_AMD_NB /
+ * LAPACK_AMD_LEAF say otherwise.  Allocates two n x n matrices while it
+ * runs.  Returns 0 or a negative LA_ERR_* code. */
+int lapack_amd_tune_getrf(int n, const char* path, int* nb, int* leaf, double* ms);
+/* Reads a table written by lapack_amd_tune_getrf (the file named by the
+ * environment variable LAPACK_AMD_TUNE_FILE is read at the first
+ * factorization); returns the number of buckets read, -1 if unreadable.
+ * A null path empties the table. */
+int lapack_amd_load_tuning(const char* path);
+/* NB and leaf width a factorization with min(m, n) = n would use. */
+void lapack_amd_get_tuning_for(int n, int* nb, int* leaf);
 
 /* XERBLA recording mode (the role of TESTING/LIN/xerbla.f): when on, the
  * default xerbla_ records the routine name and parameter number and
@@ -248,6 +350,29 @@ void lapack_amd_dlarnv(int idist, unsigned long long seed, unsigned long long of
 
 /* Library version string. */
 const char* lapack_amd_version(void);
+
+/* ============================ failure reporting =========================== */
+
+/* The multi-workgroup panel kernels of DGETRF/SGETRF and the triangular
+ * sweeps of DGETRS/SGETRS wait for each other's workgroups with bounded
+ * waits.  When a wait gives up (its workgroups were not all resident, e.g.
+ * another process held the CUs), every kernel still runs to its end and:
+ *   - getrf leaves A exactly as passed and writes INFO = LA_ERR_SYNC;
+ *   - getrs leaves B exactly as passed;
+ *   - bit 0 of the device's sticky status word is set.
+ * The synchronous entry points (Fortran drop-ins and lapack_amd_*) then redo
+ * the call once with forms that never wait (a column-at-a-time panel, a
+ * blocked triangular solve) and return its result; the mixed-precision
+ * drivers (DSGESV, DGESV_IR, DGERFS) return LA_ERR_SYNC instead.  Callers of
+ * the _async entry points read the sticky word here (the call waits for the
+ * current device to be idle); clear != 0 resets it.  Returns the word (0 =
+ * no wait gave up), or a negative LA_ERR_* code. */
+int lapack_amd_device_status(int clear);
+
+/* Test hook: the next `count` factorizations or solves start with their
+ * wait-error word already set, so each of their bounded waits gives up at
+ * once (the failure path above, forced from the host; no kernel hangs). */
+void lapack_amd_test_fail_waits(int count);
 
 #ifdef __cplusplus
 }

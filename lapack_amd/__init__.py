@@ -92,6 +92,16 @@ def _declare(L):
         "lapack_amd_phase_read": (None, [_c_dbl_p, ctypes.POINTER(ctypes.c_long), _c_dbl_p]),
         "lapack_amd_dlarnv": (None, [i, ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_longlong, vp]),
         "lapack_amd_version": (ctypes.c_char_p, []),
+        "lapack_amd_device_status": (i, [i]),
+        "lapack_amd_tune_getrf": (i, [i, ctypes.c_char_p, _c_int_p, _c_int_p, _c_dbl_p]),
+        "lapack_amd_load_tuning": (i, [ctypes.c_char_p]),
+        "lapack_amd_get_tuning_for": (None, [i, _c_int_p, _c_int_p]),
+        "lapack_amd_dlange": (d, [ch, i, i, vp, i]),
+        "lapack_amd_dgecon": (i, [ch, i, vp, i, d, _c_dbl_p]),
+        "lapack_amd_dgetri": (i, [i, vp, i, vp]),
+        "lapack_amd_dgesvx": (i, [ch, ch, i, i, vp, i, vp, i, vp, ctypes.c_char_p, vp, vp, vp, i, vp, i, _c_dbl_p,
+                                  vp, vp, _c_dbl_p]),
+        "lapack_amd_test_fail_waits": (None, [i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -187,6 +197,33 @@ def dlaqge(m, n, a, lda, r, c, rowcnd, colcnd, amax):
     """DLAQGE; returns EQUED as a str."""
     return lib().lapack_amd_dlaqge(m, n, _ptr(a), lda, _ptr(r), _ptr(c), float(rowcnd), float(colcnd),
                                    float(amax)).decode()
+
+
+def dlange(norm, m, n, a, lda):
+    """DLANGE (SRC/dlange.f:114): the norm, computed on the GPU."""
+    return lib().lapack_amd_dlange(_c(norm), m, n, _ptr(a), lda)
+
+
+def dgecon(norm, n, a, lda, anorm):
+    """DGECON (SRC/dgecon.f:130) on the factors in a; returns (INFO, RCOND)."""
+    rc = ctypes.c_double(0)
+    info = lib().lapack_amd_dgecon(_c(norm), n, _ptr(a), lda, float(anorm), ctypes.byref(rc))
+    return info, rc.value
+
+
+def dgetri(n, a, lda, ipiv):
+    """DGETRI (SRC/dgetri.f:114): A's factors replaced by inv(A); returns INFO."""
+    return lib().lapack_amd_dgetri(n, _ptr(a), lda, _ptr(ipiv))
+
+
+def dgesvx(fact, trans, n, nrhs, a, lda, af, ldaf, ipiv, equed, r, c, b, ldb, x, ldx, ferr, berr):
+    """DGESVX (SRC/dgesvx.f:349); returns (INFO, EQUED, RCOND, RPVGRW)."""
+    eq = ctypes.create_string_buffer(_c(equed), 1)
+    rcond, rpvgrw = ctypes.c_double(0), ctypes.c_double(0)
+    info = lib().lapack_amd_dgesvx(_c(fact), _c(trans), n, nrhs, _ptr(a), lda, _ptr(af), ldaf, _ptr(ipiv), eq,
+                                   _ptr(r), _ptr(c), _ptr(b), ldb, _ptr(x), ldx, ctypes.byref(rcond), _ptr(ferr),
+                                   _ptr(berr), ctypes.byref(rpvgrw))
+    return info, eq.raw.decode(), rcond.value, rpvgrw.value
 
 
 def dlaswp(n, a, lda, k1, k2, ipiv, incx):
@@ -298,6 +335,28 @@ def set_tuning(nb=0, leaf=0):
     lib().lapack_amd_set_tuning(int(nb), int(leaf))
 
 
+def tune_getrf(n, path=None):
+    """Autotune NB and the leaf width for order n; returns (nb, leaf, ms)."""
+    nb, leaf, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    rc = lib().lapack_amd_tune_getrf(int(n), path.encode() if path else None, ctypes.byref(nb), ctypes.byref(leaf),
+                                     ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"lapack_amd_tune_getrf returned {rc}")
+    return nb.value, leaf.value, ms.value
+
+
+def load_tuning(path):
+    """Read a tuning table (None empties it); returns the buckets read."""
+    return lib().lapack_amd_load_tuning(path.encode() if path else None)
+
+
+def get_tuning_for(n):
+    """(NB, leaf) a factorization of order n would use."""
+    nb, leaf = ctypes.c_int(), ctypes.c_int()
+    lib().lapack_amd_get_tuning_for(int(n), ctypes.byref(nb), ctypes.byref(leaf))
+    return nb.value, leaf.value
+
+
 def get_tuning():
     nb, leaf = ctypes.c_int(), ctypes.c_int()
     lib().lapack_amd_get_tuning(ctypes.byref(nb), ctypes.byref(leaf))
@@ -350,3 +409,15 @@ def matrix(idist, seed, m, n, offset=0):
 
 def version():
     return lib().lapack_amd_version().decode()
+
+
+def device_status(clear=True):
+    """Sticky status of the current device: bit 0 = a bounded wait between
+    workgroups gave up since the last clear (waits for the device first)."""
+    return lib().lapack_amd_device_status(1 if clear else 0)
+
+
+def test_fail_waits(count=1):
+    """Test hook: the next `count` factorizations / solves start with their
+    wait-error word set, so every bounded wait in them gives up at once."""
+    lib().lapack_amd_test_fail_waits(int(count))

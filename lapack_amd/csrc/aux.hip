@@ -10,11 +10,14 @@ namespace {
 // ---- transpose ------------------------------------------------------------
 // out(j,i) = cast(in(i,j)), 32x32 tiles through LDS (+1 padding against bank
 // conflicts), reads and writes both coalesced.
+// skip (optional): nothing is written when *skip != 0 (the factorization's
+// write-back after a bounded wait gave up leaves the caller's matrix as it was).
 template <class Ti, class To, bool CHECK>
 __global__ __launch_bounds__(256) void transpose_kernel(int rows, int cols, const Ti* __restrict__ in,
                                                         long ld_in, To* __restrict__ out, long ld_out,
-                                                        int* overflow) {
+                                                        int* overflow, const int* __restrict__ skip) {
   __shared__ Ti tile[32][33];
+  if (skip && *skip) return;
   const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
 #pragma unroll
@@ -41,10 +44,10 @@ __global__ __launch_bounds__(256) void transpose_kernel(int rows, int cols, cons
 
 template <class Ti, class To, bool CHECK>
 void launch_transpose_t(int rows, int cols, const Ti* in, long ld_in, To* out, long ld_out, int* ovf,
-                        hipStream_t s) {
+                        hipStream_t s, const int* skip = nullptr) {
   if (rows <= 0 || cols <= 0) return;
   dim3 grid((rows + 31) / 32, (cols + 31) / 32);
-  transpose_kernel<Ti, To, CHECK><<<grid, 256, 0, s>>>(rows, cols, in, ld_in, out, ld_out, ovf);
+  transpose_kernel<Ti, To, CHECK><<<grid, 256, 0, s>>>(rows, cols, in, ld_in, out, ld_out, ovf, skip);
 }
 
 // ---- row interchanges in the transposed workspace ---------------------------
@@ -589,7 +592,8 @@ __global__ __launch_bounds__(256) void swaps_to_perm_pairs_kernel(int nrows, int
 template <class T>
 __global__ __launch_bounds__(256) void gather_rows_kernel(int nrows, int ncols, const int* __restrict__ perm,
                                                           const T* __restrict__ src, long lds,
-                                                          T* __restrict__ dst, long ldd) {
+                                                          T* __restrict__ dst, long ldd, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   const long tot = (long)nrows * ncols;
   for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
     const int r = (int)(e % nrows), j = (int)(e / nrows);
@@ -670,7 +674,9 @@ __global__ __launch_bounds__(256) void trtri_kernel(int n, int nb, const T* __re
 // ---- copies and fills -----------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void copy_kernel(int rows, int cols, const T* __restrict__ src, long lds,
-                                                   T* __restrict__ dst, long ldd, int nskip) {
+                                                   T* __restrict__ dst, long ldd, int nskip,
+                                                   const int* __restrict__ skip) {
+  if (skip && *skip) return;
   // nskip: blocks dispatched to XCDs 0 .. nskip-1 (blockIdx % 8) leave at once
   long lb = blockIdx.x, ng = gridDim.x;
   if (nskip) {
@@ -684,8 +690,23 @@ __global__ __launch_bounds__(256) void copy_kernel(int rows, int cols, const T* 
     dst[i + j * ldd] = src[i + j * lds];
   }
 }
-__global__ void fold_err_kernel(const int* __restrict__ err, int* __restrict__ info) {
-  if (threadIdx.x == 0 && *err) *info = LA_ERR_SYNC;
+__global__ void fold_err_kernel(const int* __restrict__ err, int* __restrict__ info, int* __restrict__ sticky) {
+  if (threadIdx.x == 0 && *err) {
+    if (info) *info = LA_ERR_SYNC;
+    if (sticky) atomicOr(sticky, 1);
+  }
+}
+
+// x[i] uniform in [-1, 1) from a counter-based hash of (seed, i) (a
+// splitmix64 finalizer): test and tuning inputs made on the device.
+__global__ __launch_bounds__(256) void fill_random_kernel(double* __restrict__ x, size_t n, unsigned long long seed) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    unsigned long long z = seed + 0x9e3779b97f4a7c15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z = z ^ (z >> 31);
+    x[i] = (double)(z >> 11) * 0x1p-52 - 1.0;
+  }
 }
 
 template <class T>
@@ -703,12 +724,12 @@ inline int grid_for(long work) {
 }  // namespace
 
 void launch_transpose_d(int rows, int cols, const double* in, long ld_in, double* out, long ld_out,
-                        hipStream_t s) {
-  launch_transpose_t<double, double, false>(rows, cols, in, ld_in, out, ld_out, nullptr, s);
+                        hipStream_t s, const int* skip) {
+  launch_transpose_t<double, double, false>(rows, cols, in, ld_in, out, ld_out, nullptr, s, skip);
 }
 void launch_transpose_s(int rows, int cols, const float* in, long ld_in, float* out, long ld_out,
-                        hipStream_t s) {
-  launch_transpose_t<float, float, false>(rows, cols, in, ld_in, out, ld_out, nullptr, s);
+                        hipStream_t s, const int* skip) {
+  launch_transpose_t<float, float, false>(rows, cols, in, ld_in, out, ld_out, nullptr, s, skip);
 }
 void launch_transpose_d2s(int rows, int cols, const double* in, long ld_in, float* out, long ld_out,
                           int* overflow, hipStream_t s) {
@@ -798,14 +819,16 @@ void launch_swaps_to_perm(int nrows, int k1, int k2, const int* ipiv, int incx, 
   }
 }
 void launch_gather_rows_d(int nrows, int ncols, const int* perm, const double* src, long lds, double* dst,
-                          long ldd, hipStream_t s) {
+                          long ldd, hipStream_t s, const int* skip) {
   if (nrows > 0 && ncols > 0)
-    gather_rows_kernel<double><<<grid_for((long)nrows * ncols), 256, 0, s>>>(nrows, ncols, perm, src, lds, dst, ldd);
+    gather_rows_kernel<double><<<grid_for((long)nrows * ncols), 256, 0, s>>>(nrows, ncols, perm, src, lds, dst, ldd,
+                                                                             skip);
 }
 void launch_gather_rows_s(int nrows, int ncols, const int* perm, const float* src, long lds, float* dst,
-                          long ldd, hipStream_t s) {
+                          long ldd, hipStream_t s, const int* skip) {
   if (nrows > 0 && ncols > 0)
-    gather_rows_kernel<float><<<grid_for((long)nrows * ncols), 256, 0, s>>>(nrows, ncols, perm, src, lds, dst, ldd);
+    gather_rows_kernel<float><<<grid_for((long)nrows * ncols), 256, 0, s>>>(nrows, ncols, perm, src, lds, dst, ldd,
+                                                                            skip);
 }
 
 template <class T>
@@ -845,19 +868,26 @@ static int copy_grid(long work, int nskip) {
   return nskip ? (g + (8 - nskip) - 1) / (8 - nskip) * 8 : g;
 }
 void launch_copy_d(int rows, int cols, const double* src, long lds, double* dst, long ldd, hipStream_t s,
-                   int nskip) {
+                   int nskip, const int* skip) {
   if (rows > 0 && cols > 0)
-    copy_kernel<double><<<copy_grid((long)rows * cols, nskip), 256, 0, s>>>(rows, cols, src, lds, dst, ldd, nskip);
+    copy_kernel<double><<<copy_grid((long)rows * cols, nskip), 256, 0, s>>>(rows, cols, src, lds, dst, ldd, nskip,
+                                                                            skip);
 }
 void launch_copy_s(int rows, int cols, const float* src, long lds, float* dst, long ldd, hipStream_t s,
-                   int nskip) {
+                   int nskip, const int* skip) {
   if (rows > 0 && cols > 0)
-    copy_kernel<float><<<copy_grid((long)rows * cols, nskip), 256, 0, s>>>(rows, cols, src, lds, dst, ldd, nskip);
+    copy_kernel<float><<<copy_grid((long)rows * cols, nskip), 256, 0, s>>>(rows, cols, src, lds, dst, ldd, nskip,
+                                                                           skip);
 }
 void launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s) {
   if (n) fill_kernel<unsigned><<<grid_for((long)n), 256, 0, s>>>(p, n, v);
 }
-void launch_fold_err(const int* err, int* info, hipStream_t s) { fold_err_kernel<<<1, 64, 0, s>>>(err, info); }
+void launch_fill_random(double* x, size_t n, unsigned long long seed, hipStream_t s) {
+  if (n) fill_random_kernel<<<grid_for((long)std::min<size_t>(n, (size_t)1 << 40)), 256, 0, s>>>(x, n, seed);
+}
+void launch_fold_err(const int* err, int* info, int* sticky, hipStream_t s) {
+  fold_err_kernel<<<1, 64, 0, s>>>(err, info, sticky);
+}
 void launch_fill_i32(int* p, size_t n, int v, hipStream_t s) {
   if (n) fill_kernel<int><<<grid_for((long)n), 256, 0, s>>>(p, n, v);
 }

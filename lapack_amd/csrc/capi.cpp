@@ -11,9 +11,11 @@
 #include "la_common.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <vector>
 
@@ -25,10 +27,20 @@ template <class T> int getrf_device(int m, int n, T* A, long lda, int* ipiv, int
 template <class T>
 int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* ipiv, T* B, long ldb, hipStream_t s);
 template <class T>
+int getrs_device_ex(char trans, int n, int nrhs, const T* A, long lda, const int* ipiv, T* B, long ldb, hipStream_t s,
+                    int* waitflag);
+int with_safe_forms(const std::function<int()>& f);
+void fail_waits(int count);
+int device_status(int clear);
+int tune_getrf(int n, const char* path, int* nb, int* leaf, double* ms);
+int load_tuning(const char* path);
+template <class T>
 int laswp_device(int ncols, T* A, long lda, int nrows, int k1, int k2, const int* ipiv, int incx, hipStream_t s);
 void set_tuning(int nb, int leaf);
 int tune_nb();
 int tune_leaf();
+int tune_nb_for(int n);
+int tune_leaf_for(int n);
 DevCtx* ctx_for_current();
 hipStream_t ctx_stream(DevCtx* c);
 std::mutex& io_mutex(DevCtx* c);
@@ -48,6 +60,9 @@ char laqge_device(int m, int n, double* A, long lda, const double* d_r, const do
 int gerfs_device(char trans, int n, int nrhs, const double* A, long lda, const double* AF, long ldaf,
                  const int* ipiv, const double* B, long ldb, double* X, long ldx, double* ferr, double* berr,
                  hipStream_t s);
+int gecon_device(bool onenrm, int n, const double* AF, long ldaf, double anorm, double* rcond, hipStream_t s);
+int getri_device(int n, double* A, long lda, const int* ipiv, hipStream_t s);
+int lange_device(char norm, int m, int n, const double* A, long lda, double* value, hipStream_t s);
 }  // namespace la
 
 using namespace la;
@@ -165,13 +180,20 @@ int getrf_sync(int m, int n, T* a, int lda, int* ipiv) {
   place(A, cur);
   place(P, cur);
   if ((rc = to_dev(A, call.s))) return rc;
-  rc = getrf_device<T>(m, n, A.dev, A.ld, P.dev, dinfo, call.s);
-  if (rc) return rc;
-  if ((rc = to_host(A, call.s))) return rc;
-  if ((rc = to_host(P, call.s))) return rc;
   int info = 0;
-  CK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, call.s));
-  CK(hipStreamSynchronize(call.s));
+  auto factor = [&]() -> int {
+    int r = getrf_device<T>(m, n, A.dev, A.ld, P.dev, dinfo, call.s);
+    if (r) return r;
+    if ((r = to_host(A, call.s))) return r;
+    if ((r = to_host(P, call.s))) return r;
+    CK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, call.s));
+    CK(hipStreamSynchronize(call.s));
+    return 0;
+  };
+  if ((rc = factor())) return rc;
+  // a bounded wait gave up: A was left as passed; factor it again with the
+  // panel form that has no waits
+  if (info == LA_ERR_SYNC && (rc = with_safe_forms(factor))) return rc;
   return info;
 }
 
@@ -198,14 +220,26 @@ int getrs_sync(char trans, int n, int nrhs, const T* a, int lda, const int* ipiv
   char* io = nullptr;
   if ((rc = io_buffer(call.c, A.bytes() + P.bytes() + B.bytes() + 256, &io))) return rc;
   char* cur = io;
+  int* dflag = reinterpret_cast<int*>(cur);
+  cur += 256;
   place(A, cur);
   place(P, cur);
   place(B, cur);
   if ((rc = to_dev(A, call.s)) || (rc = to_dev(P, call.s)) || (rc = to_dev(B, call.s))) return rc;
-  if ((rc = getrs_device<T>(trans, n, nrhs, A.dev, A.ld, P.dev, B.dev, B.ld, call.s))) return rc;
-  if ((rc = to_host(B, call.s))) return rc;
-  CK(hipStreamSynchronize(call.s));
-  return 0;
+  int flag = 0;
+  auto solve = [&]() -> int {
+    launch_fill_i32(dflag, 1, 0, call.s);
+    int r = getrs_device_ex<T>(trans, n, nrhs, A.dev, A.ld, P.dev, B.dev, B.ld, call.s, dflag);
+    if (r) return r;
+    if ((r = to_host(B, call.s))) return r;
+    CK(hipMemcpyAsync(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost, call.s));
+    CK(hipStreamSynchronize(call.s));
+    return 0;
+  };
+  if ((rc = solve())) return rc;
+  // a bounded wait gave up: B was left as passed; solve again without waits
+  if (flag && (rc = with_safe_forms(solve))) return rc;
+  return flag ? LA_ERR_SYNC : 0;
 }
 
 int gesv_sync(int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb) {
@@ -229,13 +263,31 @@ int gesv_sync(int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb
   place(P, cur);
   place(B, cur);
   if ((rc = to_dev(A, call.s)) || (rc = to_dev(B, call.s))) return rc;
-  if ((rc = getrf_device<double>(n, n, A.dev, A.ld, P.dev, dinfo, call.s))) return rc;
-  int info = 0;
-  CK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, call.s));
-  CK(hipStreamSynchronize(call.s));
+  int info = 0, flag = 0;
+  auto factor = [&]() -> int {
+    int r = getrf_device<double>(n, n, A.dev, A.ld, P.dev, dinfo, call.s);
+    if (r) return r;
+    CK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, call.s));
+    CK(hipStreamSynchronize(call.s));
+    return 0;
+  };
+  auto solve = [&]() -> int {
+    launch_fill_i32(dinfo + 1, 1, 0, call.s);
+    int r = getrs_device_ex<double>('N', n, nrhs, A.dev, A.ld, P.dev, B.dev, B.ld, call.s, dinfo + 1);
+    if (r) return r;
+    CK(hipMemcpyAsync(&flag, dinfo + 1, sizeof(int), hipMemcpyDeviceToHost, call.s));
+    CK(hipStreamSynchronize(call.s));
+    return 0;
+  };
+  if ((rc = factor())) return rc;
+  // a bounded wait gave up: A was left as passed; factor it again without waits
+  if (info == LA_ERR_SYNC && (rc = with_safe_forms(factor))) return rc;
   // DGESV solves only when the factorization is nonsingular (SRC/dgesv.f:170-175).
-  if (info == 0 && nrhs > 0)
-    if ((rc = getrs_device<double>('N', n, nrhs, A.dev, A.ld, P.dev, B.dev, B.ld, call.s))) return rc;
+  if (info == 0 && nrhs > 0) {
+    if ((rc = solve())) return rc;
+    if (flag && (rc = with_safe_forms(solve))) return rc;
+    if (flag) info = LA_ERR_SYNC;
+  }
   if ((rc = to_host(A, call.s)) || (rc = to_host(P, call.s))) return rc;
   if (info == 0 && (rc = to_host(B, call.s))) return rc;
   CK(hipStreamSynchronize(call.s));
@@ -271,6 +323,8 @@ int dsgesv_sync(int n, int nrhs, double* a, int lda, int* ipiv, const double* b,
   if (info < 0) return info;
   if ((rc = to_host(A, call.s)) || (rc = to_host(P, call.s)) || (rc = to_host(X, call.s))) return rc;
   CK(hipStreamSynchronize(call.s));
+  // a bounded wait inside gave up: the results are not to be trusted
+  if (device_status(1) > 0) return LA_ERR_SYNC;
   return info;
 }
 
@@ -311,6 +365,7 @@ int dgesv_ir_sync(int n, int nrhs, double* a, int lda, int* ipiv, const double* 
   if (info < 0) return info;
   if ((rc = to_host(A, call.s)) || (rc = to_host(P, call.s)) || (rc = to_host(X, call.s))) return rc;
   CK(hipStreamSynchronize(call.s));
+  if (device_status(1) > 0) return LA_ERR_SYNC;
   return info;
 }
 
@@ -354,6 +409,7 @@ int gerfs_sync(char trans, int n, int nrhs, const double* a, int lda, const doub
       return rc;
     if ((rc = to_host(X, call.s))) return rc;
     CK(hipStreamSynchronize(call.s));
+    if (device_status(1) > 0) return LA_ERR_SYNC;
   }
   if (nrhs > 0) {
     if (fdev) CK(hipMemcpy(ferr, hf.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice));
@@ -362,6 +418,286 @@ int gerfs_sync(char trans, int n, int nrhs, const double* a, int lda, const doub
     else std::memcpy(berr, hb.data(), sizeof(double) * nrhs);
   }
   return 0;
+}
+
+// A host-or-device array of `count` doubles read into / written from host memory.
+int get_hd(const double* p, double* out, int count) {
+  if (count <= 0) return 0;
+  if (is_device_ptr(p)) CK(hipMemcpy(out, p, sizeof(double) * count, hipMemcpyDeviceToHost));
+  else std::memcpy(out, p, sizeof(double) * count);
+  return 0;
+}
+int put_hd(double* p, const double* in, int count) {
+  if (count <= 0) return 0;
+  if (is_device_ptr(p)) CK(hipMemcpy(p, in, sizeof(double) * count, hipMemcpyHostToDevice));
+  else std::memcpy(p, in, sizeof(double) * count);
+  return 0;
+}
+
+// DLANGE (SRC/dlange.f:114-200): returns the norm; *err receives a library
+// error code (the reference has no error exit).
+double lange_sync(char norm, int m, int n, const double* a, int lda, int* err) {
+  *err = 0;
+  if (std::min(m, n) <= 0) return 0.0;
+  Call call;
+  int rc = call.open();
+  if (rc) { *err = rc; return 0.0; }
+  Arr<double> A = arr<double>(const_cast<double*>(a), lda, m, n);
+  char* io = nullptr;
+  if ((rc = io_buffer(call.c, A.bytes() + 256, &io))) { *err = rc; return 0.0; }
+  char* cur = io;
+  place(A, cur);
+  if ((rc = to_dev(A, call.s))) { *err = rc; return 0.0; }
+  double v = 0.0;
+  if ((rc = lange_device(norm, m, n, A.dev, A.ld, &v, call.s))) *err = rc;
+  return v;
+}
+
+// DGECON (argument checks SRC/dgecon.f:157-185).
+int gecon_sync(char norm, int n, const double* a, int lda, double anorm, double* rcond) {
+  const bool onenrm = norm == '1' || lsame(norm, 'O');
+  if (!onenrm && !lsame(norm, 'I')) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max(1, n)) return -4;
+  if (anorm < 0.0) return -5;
+  *rcond = 0.0;
+  if (n == 0) {
+    *rcond = 1.0;
+    return 0;
+  }
+  if (anorm == 0.0) return 0;
+  if (std::isnan(anorm)) {
+    *rcond = anorm;
+    return -5;
+  }
+  if (anorm > 1.79769313486231571e308) return -5;
+  Call call;
+  int rc = call.open();
+  if (rc) return rc;
+  Arr<double> A = arr<double>(const_cast<double*>(a), lda, n, n);
+  char* io = nullptr;
+  if ((rc = io_buffer(call.c, A.bytes() + 256, &io))) return rc;
+  char* cur = io;
+  place(A, cur);
+  if ((rc = to_dev(A, call.s))) return rc;
+  return gecon_device(onenrm, n, A.dev, A.ld, anorm, rcond, call.s);
+}
+
+// DGETRI (argument checks SRC/dgetri.f:141-158); WORK(1) = N*NB (NB = 64,
+// the reference's ILAENV value) as the reference reports its optimal size.
+int getri_sync(int n, double* a, int lda, const int* ipiv, double* work, int lwork) {
+  const double lwkopt = (double)std::max(1, n * 64);
+  if (work) put_hd(work, &lwkopt, 1);
+  const bool lquery = lwork == -1;
+  if (n < 0) return -1;
+  if (lda < std::max(1, n)) return -3;
+  if (lwork < std::max(1, n) && !lquery) return -6;
+  if (lquery || n == 0) return 0;
+  Call call;
+  int rc = call.open();
+  if (rc) return rc;
+  Arr<double> A = arr<double>(a, lda, n, n);
+  Arr<int> P = arr<int>(const_cast<int*>(ipiv), n, n, 1);
+  char* io = nullptr;
+  if ((rc = io_buffer(call.c, A.bytes() + P.bytes() + 256, &io))) return rc;
+  char* cur = io;
+  place(A, cur);
+  place(P, cur);
+  if ((rc = to_dev(A, call.s)) || (rc = to_dev(P, call.s))) return rc;
+  const int info = getri_device(n, A.dev, A.ld, P.dev, call.s);
+  if (info < 0) return info;
+  if (info == 0 && (rc = to_host(A, call.s))) return rc;
+  CK(hipStreamSynchronize(call.s));
+  return info;
+}
+
+// DGESVX (SRC/dgesvx.f:344-600): the reference's sequence -- equilibration
+// (DGEEQU / DLAQGE), DGETRF on a copy, the reciprocal pivot growth, DGECON,
+// DGETRS, DGERFS, the back-scaling of X and FERR, INFO = N+1 below eps --
+// with every O(n^2) and O(n^3) step on the GPU and the scalar decisions on
+// the host.  Matrices may be host or device memory; R, C, FERR, BERR, WORK
+// and RCOND too.  Returns INFO; *equed as the reference sets it.
+int gesvx_sync(char fact, char trans, int n, int nrhs, double* a, int lda, double* af, int ldaf, int* ipiv,
+               char* equed, double* r, double* c, double* b, int ldb, double* x, int ldx, double* rcond,
+               double* ferr, double* berr, double* work) {
+  const bool nofact = lsame(fact, 'N'), equil = lsame(fact, 'E'), notran = lsame(trans, 'N');
+  const double smlnum = 0x1p-1022, bignum = 1.0 / smlnum, eps = 0x1p-53;
+  bool rowequ = false, colequ = false;
+  double rowcnd = 1.0, colcnd = 1.0, amax = 0.0;
+  std::vector<double> hr(std::max(n, 1)), hc(std::max(n, 1));
+  if (nofact || equil) {
+    *equed = 'N';
+  } else {
+    rowequ = lsame(*equed, 'R') || lsame(*equed, 'B');
+    colequ = lsame(*equed, 'C') || lsame(*equed, 'B');
+  }
+  if (!nofact && !equil && !lsame(fact, 'F')) return -1;
+  if (!notran && !lsame(trans, 'T') && !lsame(trans, 'C')) return -2;
+  if (n < 0) return -3;
+  if (nrhs < 0) return -4;
+  if (lda < std::max(1, n)) return -6;
+  if (ldaf < std::max(1, n)) return -8;
+  if (lsame(fact, 'F') && !(rowequ || colequ || lsame(*equed, 'N'))) return -10;
+  int rc;
+  if (rowequ) {
+    if ((rc = get_hd(r, hr.data(), n))) return rc;
+    double rcmin = bignum, rcmax = 0.0;
+    for (int j = 0; j < n; ++j) { rcmin = std::min(rcmin, hr[j]); rcmax = std::max(rcmax, hr[j]); }
+    if (rcmin <= 0.0) return -11;
+    rowcnd = n > 0 ? std::max(rcmin, smlnum) / std::min(rcmax, bignum) : 1.0;
+  }
+  if (colequ) {
+    if ((rc = get_hd(c, hc.data(), n))) return rc;
+    double rcmin = bignum, rcmax = 0.0;
+    for (int j = 0; j < n; ++j) { rcmin = std::min(rcmin, hc[j]); rcmax = std::max(rcmax, hc[j]); }
+    if (rcmin <= 0.0) return -12;
+    colcnd = n > 0 ? std::max(rcmin, smlnum) / std::min(rcmax, bignum) : 1.0;
+  }
+  if (ldb < std::max(1, n)) return -14;
+  if (ldx < std::max(1, n)) return -16;
+  std::vector<double> hf(std::max(nrhs, 1), 0.0), hb(std::max(nrhs, 1), 0.0);
+  if (n == 0) {
+    // quick return as the reference's zero-order path: RCOND from DGECON = 1,
+    // FERR = BERR = 0 from DGERFS, RPVGRW = 1
+    double one = 1.0;
+    put_hd(rcond, &one, 1);
+    put_hd(ferr, hf.data(), nrhs);
+    put_hd(berr, hb.data(), nrhs);
+    if (work) put_hd(work, &one, 1);
+    return 0;
+  }
+  Call call;
+  if ((rc = call.open())) return rc;
+  Arr<double> A = arr<double>(a, lda, n, n);
+  Arr<double> F = arr<double>(af, ldaf, n, n);
+  Arr<int> P = arr<int>(ipiv, n, n, 1);
+  Arr<double> B = arr<double>(b, ldb, n, nrhs);
+  Arr<double> X = arr<double>(x, ldx, n, nrhs);
+  char* io = nullptr;
+  if ((rc = io_buffer(call.c, A.bytes() + F.bytes() + P.bytes() + B.bytes() + X.bytes() +
+                                  2 * al256(sizeof(double) * (size_t)n) + 256, &io)))
+    return rc;
+  char* cur = io;
+  int* dinfo = reinterpret_cast<int*>(cur);
+  cur += 256;
+  double* d_r = reinterpret_cast<double*>(cur);
+  cur += al256(sizeof(double) * (size_t)n);
+  double* d_c = reinterpret_cast<double*>(cur);
+  cur += al256(sizeof(double) * (size_t)n);
+  place(A, cur);
+  place(F, cur);
+  place(P, cur);
+  place(B, cur);
+  place(X, cur);
+  hipStream_t s = call.s;
+  if ((rc = to_dev(A, s)) || (rc = to_dev(B, s))) return rc;
+  if (!nofact && !equil && ((rc = to_dev(F, s)) || (rc = to_dev(P, s)))) return rc;
+  bool a_changed = false, b_changed = false;
+  if (equil) {
+    const int infequ = geequ_device(n, n, A.dev, A.ld, hr.data(), hc.data(), &rowcnd, &colcnd, &amax, 0, d_r, d_c,
+                                    s);
+    if (infequ < 0) return infequ;
+    // DGEEQU leaves R (and C) as the reference leaves them
+    if ((rc = put_hd(r, hr.data(), n))) return rc;
+    if ((infequ == 0 || infequ > n) && (rc = put_hd(c, hc.data(), n))) return rc;
+    if (infequ == 0) {
+      *equed = laqge_device(n, n, A.dev, A.ld, d_r, d_c, rowcnd, colcnd, amax, s);
+      a_changed = *equed != 'N';
+      rowequ = lsame(*equed, 'R') || lsame(*equed, 'B');
+      colequ = lsame(*equed, 'C') || lsame(*equed, 'B');
+    }
+  } else {
+    if (rowequ) CK(hipMemcpyAsync(d_r, hr.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (colequ) CK(hipMemcpyAsync(d_c, hc.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  }
+  // B <- diag(R) B or diag(C) B (dgesvx.f:411-425)
+  if (notran && rowequ) { launch_scale_rc(n, nrhs, B.dev, B.ld, d_r, nullptr, 1, s); b_changed = true; }
+  if (!notran && colequ) { launch_scale_rc(n, nrhs, B.dev, B.ld, d_c, nullptr, 1, s); b_changed = true; }
+  auto copy_back = [&](bool with_af, bool with_x) -> int {
+    int r2;
+    if (a_changed && (r2 = to_host(A, s))) return r2;
+    if (b_changed && (r2 = to_host(B, s))) return r2;
+    if (with_af && ((r2 = to_host(F, s)) || (r2 = to_host(P, s)))) return r2;
+    if (with_x && (r2 = to_host(X, s))) return r2;
+    CK(hipStreamSynchronize(s));
+    return 0;
+  };
+  auto upper_max = [&](const double* M, long ld, int rows, int cols, int mode, double* out) -> int {
+    double* d_v = d_r;  // n doubles of scratch once R has been used
+    std::vector<double> h(cols);
+    launch_col_norm(rows, cols, M, ld, mode, 1.0, d_v, s);
+    CK(hipMemcpyAsync(h.data(), d_v, sizeof(double) * cols, hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    double v = 0.0;
+    for (int j = 0; j < cols; ++j)
+      if (v < h[j] || std::isnan(h[j])) v = h[j];
+    *out = v;
+    return 0;
+  };
+  int info = 0;
+  if (nofact || equil) {
+    launch_copy_d(n, n, A.dev, A.ld, F.dev, F.ld, s);
+    auto factor = [&]() -> int {
+      int r2 = getrf_device<double>(n, n, F.dev, F.ld, P.dev, dinfo, s);
+      if (r2) return r2;
+      CK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+      CK(hipStreamSynchronize(s));
+      return 0;
+    };
+    if ((rc = factor())) return rc;
+    if (info == LA_ERR_SYNC && (rc = with_safe_forms(factor))) return rc;
+    if (info < 0) return info;
+    if (info > 0) {
+      // the reciprocal pivot growth of the leading INFO columns (dgesvx.f:437-450)
+      double rpvgrw = 0.0, am = 0.0;
+      if ((rc = upper_max(F.dev, F.ld, info, info, 3, &rpvgrw))) return rc;
+      if (rpvgrw == 0.0) {
+        rpvgrw = 1.0;
+      } else {
+        if ((rc = upper_max(A.dev, A.ld, n, info, 0, &am))) return rc;
+        rpvgrw = am / rpvgrw;
+      }
+      if (work && (rc = put_hd(work, &rpvgrw, 1))) return rc;
+      const double zero = 0.0;
+      if ((rc = put_hd(rcond, &zero, 1))) return rc;
+      if ((rc = copy_back(true, false))) return rc;
+      return info;
+    }
+  }
+  double anorm = 0.0, rpvgrw = 0.0, amx = 0.0;
+  if ((rc = lange_device(notran ? '1' : 'I', n, n, A.dev, A.ld, &anorm, s))) return rc;
+  if ((rc = upper_max(F.dev, F.ld, n, n, 3, &rpvgrw))) return rc;
+  if (rpvgrw == 0.0) {
+    rpvgrw = 1.0;
+  } else {
+    if ((rc = upper_max(A.dev, A.ld, n, n, 0, &amx))) return rc;
+    rpvgrw = amx / rpvgrw;
+  }
+  double rc_est = 0.0;
+  rc = gecon_device(notran, n, F.dev, F.ld, anorm, &rc_est, s);
+  if (rc < 0) return rc;
+  launch_copy_d(n, nrhs, B.dev, B.ld, X.dev, X.ld, s);
+  if ((rc = getrs_device<double>(notran ? 'N' : 'T', n, nrhs, F.dev, F.ld, P.dev, X.dev, X.ld, s))) return rc;
+  if ((rc = gerfs_device(notran ? 'N' : 'T', n, nrhs, A.dev, A.ld, F.dev, F.ld, P.dev, B.dev, B.ld, X.dev, X.ld,
+                         hf.data(), hb.data(), s)))
+    return rc;
+  // X <- diag(C) X, FERR / COLCND (or R, ROWCND for the transposed system)
+  if (notran && colequ) {
+    launch_scale_rc(n, nrhs, X.dev, X.ld, d_c, nullptr, 1, s);
+    for (int j = 0; j < nrhs; ++j) hf[j] = hf[j] / colcnd;
+  } else if (!notran && rowequ) {
+    CK(hipMemcpyAsync(d_r, hr.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    launch_scale_rc(n, nrhs, X.dev, X.ld, d_r, nullptr, 1, s);
+    for (int j = 0; j < nrhs; ++j) hf[j] = hf[j] / rowcnd;
+  }
+  if ((rc = copy_back(nofact || equil, true))) return rc;
+  if (device_status(1) > 0) return LA_ERR_SYNC;
+  if ((rc = put_hd(rcond, &rc_est, 1)) || (rc = put_hd(ferr, hf.data(), nrhs)) ||
+      (rc = put_hd(berr, hb.data(), nrhs)))
+    return rc;
+  if (work && (rc = put_hd(work, &rpvgrw, 1))) return rc;
+  if (rc_est < eps) info = n + 1;
+  return info;
 }
 
 // DGEEQU / DGEEQUB (SRC/dgeequ.f:163-173, SRC/dgeequb.f:170-180 argument checks).
@@ -608,6 +944,27 @@ void dlaqge_(const int* m, const int* n, double* a, const int* lda, const double
   int err = 0;
   *equed = laqge_sync(*m, *n, a, *lda, r, c, *rowcnd, *colcnd, *amax, &err);
 }
+double dlange_(const char* norm, const int* m, const int* n, const double* a, const int* lda, double*, size_t) {
+  int err = 0;
+  return lange_sync(*norm, *m, *n, a, *lda, &err);
+}
+void dgecon_(const char* norm, const int* n, const double* a, const int* lda, const double* anorm, double* rcond,
+             double*, int*, int* info, size_t) {
+  *info = gecon_sync(*norm, *n, a, *lda, *anorm, rcond);
+  if (*info < 0 && *info > -1000) call_xerbla("DGECON", *info);
+}
+void dgetri_(const int* n, double* a, const int* lda, const int* ipiv, double* work, const int* lwork, int* info) {
+  *info = getri_sync(*n, a, *lda, ipiv, work, *lwork);
+  if (*info < 0 && *info > -1000) call_xerbla("DGETRI", *info);
+}
+void dgesvx_(const char* fact, const char* trans, const int* n, const int* nrhs, double* a, const int* lda, double* af,
+             const int* ldaf, int* ipiv, char* equed, double* r, double* c, double* b, const int* ldb, double* x,
+             const int* ldx, double* rcond, double* ferr, double* berr, double* work, int*, int* info, size_t, size_t,
+             size_t) {
+  *info = gesvx_sync(*fact, *trans, *n, *nrhs, a, *lda, af, *ldaf, ipiv, equed, r, c, b, *ldb, x, *ldx, rcond, ferr,
+                     berr, work);
+  if (*info < 0 && *info > -1000) call_xerbla("DGESVX", *info);
+}
 void dlaswp_(const int* n, double* a, const int* lda, const int* k1, const int* k2, const int* ipiv,
              const int* incx) {
   (void)laswp_sync<double>(*n, a, *lda, *k1, *k2, ipiv, *incx);
@@ -655,6 +1012,24 @@ char lapack_amd_dlaqge(int m, int n, double* a, int lda, const double* r, const 
                        double colcnd, double amax) {
   int err = 0;
   return laqge_sync(m, n, a, lda, r, c, rowcnd, colcnd, amax, &err);
+}
+double lapack_amd_dlange(char norm, int m, int n, const double* a, int lda) {
+  int err = 0;
+  const double v = lange_sync(norm, m, n, a, lda, &err);
+  return err ? std::nan("") : v;
+}
+int lapack_amd_dgecon(char norm, int n, const double* a, int lda, double anorm, double* rcond) {
+  return gecon_sync(norm, n, a, lda, anorm, rcond);
+}
+int lapack_amd_dgetri(int n, double* a, int lda, const int* ipiv) {
+  double w = 0.0;
+  return getri_sync(n, a, lda, ipiv, &w, std::max(1, n));
+}
+int lapack_amd_dgesvx(char fact, char trans, int n, int nrhs, double* a, int lda, double* af, int ldaf, int* ipiv,
+                      char* equed, double* r, double* c, double* b, int ldb, double* x, int ldx, double* rcond,
+                      double* ferr, double* berr, double* rpvgrw) {
+  return gesvx_sync(fact, trans, n, nrhs, a, lda, af, ldaf, ipiv, equed, r, c, b, ldb, x, ldx, rcond, ferr, berr,
+                    rpvgrw);
 }
 int lapack_amd_dlaswp(int n, double* a, int lda, int k1, int k2, const int* ipiv, int incx) {
   return laswp_sync<double>(n, a, lda, k1, k2, ipiv, incx);
@@ -743,6 +1118,16 @@ void lapack_amd_get_tuning(int* nb, int* leaf) {
 void lapack_amd_phase_enable(int on) { phase_enable(on); }
 void lapack_amd_phase_reset(void) { phase_reset(); }
 void lapack_amd_phase_read(double* ms, long* calls, double* flops) { phase_read(ms, calls, flops); }
-const char* lapack_amd_version(void) { return "lapack_amd 0.1 (gfx950)"; }
+const char* lapack_amd_version(void) { return "lapack_amd 0.2 (gfx950)"; }
+int lapack_amd_device_status(int clear) { return device_status(clear); }
+int lapack_amd_tune_getrf(int n, const char* path, int* nb, int* leaf, double* ms) {
+  return tune_getrf(n, path, nb, leaf, ms);
+}
+int lapack_amd_load_tuning(const char* path) { return load_tuning(path); }
+void lapack_amd_get_tuning_for(int n, int* nb, int* leaf) {
+  if (nb) *nb = tune_nb_for(n);
+  if (leaf) *leaf = tune_leaf_for(n);
+}
+void lapack_amd_test_fail_waits(int count) { fail_waits(count); }
 
 }  // extern "C"

@@ -26,6 +26,9 @@
 //     x = inv(M_dd) y, one wavefront per four right-hand sides;
 //   * x is stored with device-coherent stores, the stores are waited for,
 //     and the block's flag is raised.
+// Every wait is bounded; one that gives up sets *err, and the other waits
+// poll *err and give up as well, so every workgroup runs to its end (the
+// caller then leaves B as it was and reports the failure).
 #include "la_common.h"
 
 namespace la {
@@ -164,7 +167,10 @@ __global__ __launch_bounds__(256) void tri_chain_kernel(int n, int nrhs, const T
       unsigned spins = 0;
       while (ld_dev(&flags[j]) == 0u) {
         __builtin_amdgcn_s_sleep(2);
-        if (++spins > (1u << 22)) { st_dev(err, 1); failed = true; break; }
+        ++spins;
+        // a wait that gave up anywhere ends every other wait too
+        if ((spins & 255u) == 1u && ld_dev(err) != 0) { failed = true; break; }
+        if (spins > (1u << 22)) { st_dev(err, 1); failed = true; break; }
       }
     }
     __syncthreads();

@@ -52,10 +52,11 @@ void launch_sgemm(char ta, char tb, int m, int n, int k, float alpha, const floa
                   const float* B, long ldb, float beta, float* C, long ldc, hipStream_t s, int nskip = 0);
 
 // out(j,i) = in(i,j): in is rows x cols with ld_in, out cols x rows with ld_out.
+// skip (optional device int): nothing is written when *skip != 0.
 void launch_transpose_d(int rows, int cols, const double* in, long ld_in, double* out, long ld_out,
-                        hipStream_t s);
+                        hipStream_t s, const int* skip = nullptr);
 void launch_transpose_s(int rows, int cols, const float* in, long ld_in, float* out, long ld_out,
-                        hipStream_t s);
+                        hipStream_t s, const int* skip = nullptr);
 // Cast/transposing variants used by the mixed-precision solver.
 void launch_transpose_d2s(int rows, int cols, const double* in, long ld_in, float* out, long ld_out,
                           int* overflow, hipStream_t s);
@@ -86,10 +87,11 @@ void launch_apply_pairs_rows_s(float* B, long ldb, int ncols, const int* pairs, 
 void launch_swaps_to_perm(int nrows, int k1, int k2, const int* ipiv, int incx, int* perm, hipStream_t s,
                           int* gp = nullptr);
 size_t perm_pairs_ints(int k1, int k2);  // device scratch of launch_swaps_to_perm's two-launch form
+// skip (optional device int): nothing is written when *skip != 0.
 void launch_gather_rows_d(int nrows, int ncols, const int* perm, const double* src, long lds, double* dst,
-                          long ldd, hipStream_t s);
+                          long ldd, hipStream_t s, const int* skip = nullptr);
 void launch_gather_rows_s(int nrows, int ncols, const int* perm, const float* src, long lds, float* dst,
-                          long ldd, hipStream_t s);
+                          long ldd, hipStream_t s, const int* skip = nullptr);
 
 // Batched triangular block inverses along the diagonal of an n x n matrix:
 // block d (rows/cols d*nb..) of tri(U) inverted into W(:, d*nb..), nb <= 256.
@@ -107,28 +109,35 @@ void launch_trtri_un_s(int nb, const float* U, long ldu, float* W, long ldw, hip
 // Copy a rows x cols column-major block.
 // nskip (copy, interchanges, GEMM, update): no work on XCDs 0 .. nskip-1,
 // which a concurrent panel keeps for itself.
+// skip (optional device int): nothing is written when *skip != 0.
 void launch_copy_d(int rows, int cols, const double* src, long lds, double* dst, long ldd, hipStream_t s,
-                   int nskip = 0);
+                   int nskip = 0, const int* skip = nullptr);
 void launch_copy_s(int rows, int cols, const float* src, long lds, float* dst, long ldd, hipStream_t s,
-                   int nskip = 0);
+                   int nskip = 0, const int* skip = nullptr);
 void launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s);
 void launch_fill_i32(int* p, size_t n, int v, hipStream_t s);
-// info <- LA_ERR_SYNC when a panel barrier timed out (err != 0).
-void launch_fold_err(const int* err, int* info, hipStream_t s);
+// x[i] uniform in [-1, 1) from a counter-based hash of (seed, i)
+void launch_fill_random(double* x, size_t n, unsigned long long seed, hipStream_t s);
+// When a bounded wait gave up (*err != 0): *info <- LA_ERR_SYNC and bit 0 of
+// *sticky set (either pointer may be null).
+void launch_fold_err(const int* err, int* info, int* sticky, hipStream_t s);
 
-// Cooperative panel (leaf) factorization on the transposed workspace: see
-// panel.hip.  tags (panel_tag_count() unsigned) must be zero at the start of
-// a factorization and tagbase must grow by >= 64 from one launch to the next.
+// Panel (leaf) factorization on the transposed workspace: see panel.hip.
+// xchg (panel_xchg_bytes() bytes) must be zero at the start of a
+// factorization and tagbase must grow by >= 64 from one launch to the next.
 int panel_max_width(int h);  // widest leaf the panel kernel takes at height h
 size_t panel_xchg_bytes(int max_groups);
-size_t panel_tag_count();
 int panel_groups(int h);     // workgroups used at height h (0 = one workgroup, LDS only)
-// rankctr: a zeroed counter private to this launch; xcd0: run the
-// multi-workgroup form on XCD 0 only (when it fits one workgroup per CU).
+// mode of a leaf taller than one workgroup holds:
+enum {
+  PANEL_PLAIN = 0,  // G register-resident workgroups anywhere on the device
+  PANEL_XCD0 = 1,   // the same on XCD 0 only (8*G blocks launched, every eighth takes part)
+  PANEL_TALL = 2,   // column by column over HBM, no workgroup waits for another
+};
 int launch_getf2_panel_d(double* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned tagbase,
-                         double* xchg, unsigned* tags, int* err, unsigned* rankctr, bool xcd0, hipStream_t s);
+                         double* xchg, int* err, int mode, hipStream_t s);
 int launch_getf2_panel_s(float* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned tagbase,
-                         double* xchg, unsigned* tags, int* err, unsigned* rankctr, bool xcd0, hipStream_t s);
+                         double* xchg, int* err, int mode, hipStream_t s);
 
 // Mixed-precision helpers (mp.hip).
 void launch_cast_d2s(int m, int n, const double* A, long lda, float* S, long lds, int* ovf, hipStream_t s);
@@ -139,6 +148,10 @@ void launch_row_abssum(int m, int n, const double* A, long lda, double* rowsum, 
 void launch_row_absmax(int m, int n, const double* A, long lda, double* r, hipStream_t s);
 void launch_col_absmax_scaled(int m, int n, const double* A, long lda, const double* r, double* c, hipStream_t s);
 void launch_scale_rc(int m, int n, double* A, long lda, const double* r, const double* c, int mode, hipStream_t s);
+void launch_identity(int n, double* A, long lda, hipStream_t s);  // A <- I
+// out[j] for each column: mode 0 max|a| (NaN-propagating), 1 sum|a|, 2 sum (a/scale)^2,
+// 3 max|a| over rows 0..j (upper triangle)
+void launch_col_norm(int m, int n, const double* A, long lda, int mode, double scale, double* out, hipStream_t s);
 void launch_scale_cast_d2s(int m, int n, const double* A, long lda, const double* r, const double* c, float* S,
                            long lds, int* ovf, hipStream_t s);
 void launch_scale_s2d(int m, int n, const float* S, long lds, const double* c, double* Y, long ldy, int add,

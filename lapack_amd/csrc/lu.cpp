@@ -27,8 +27,12 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cctype>
 #include <cstring>
+#include <functional>
+#include <map>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/lapack_amd.h"
@@ -51,6 +55,8 @@ struct DevCtx {
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   char* mp = nullptr;            // buffers of the mixed-precision solvers
   size_t mp_bytes = 0;
+  int* status = nullptr;         // sticky device word: bit 0 = a bounded wait gave up
+  bool xcd8 = false;             // 8 XCDs of >= 32 CUs: the panel may own XCD 0
   std::mutex mu;     // guards ws and the launch sequence of one call
   std::mutex io_mu;  // guards io across a whole synchronous call
   std::mutex mp_mu;  // guards mp across a whole mixed-precision solve
@@ -60,6 +66,14 @@ static std::mutex g_mu;
 static std::vector<DevCtx*> g_ctx;
 static std::atomic<int> g_nb{0}, g_leaf{0};
 static thread_local int t_last_error = 0;
+// Set while a synchronous entry point redoes a call whose bounded wait gave
+// up: every leaf then takes the panel form without waits and DGETRS the
+// blocked form without waits.
+static thread_local int t_safe = 0;
+// Test hook: the next g_fail_waits factorizations / solves start with their
+// wait-error word already set, so every bounded wait in them gives up at
+// once (each kernel still runs to its end).
+static std::atomic<int> g_fail_waits{0};
 
 static int env_int(const char* name, int dflt) {
   const char* v = std::getenv(name);
@@ -67,15 +81,112 @@ static int env_int(const char* name, int dflt) {
   return std::atoi(v);
 }
 
-int tune_nb() {
+// ---- block size NB and panel leaf width ---------------------------------------
+// In order: lapack_amd_set_tuning (the XLAENV analogue), the environment
+// (LAPACK_AMD_NB, LAPACK_AMD_LEAF), the tuned table for the order's bucket
+// (lapack_amd_tune_getrf / lapack_amd_load_tuning, or the file named by
+// LAPACK_AMD_TUNE_FILE, read once), then the defaults 256 / 64.
+struct TuneEntry {
+  int nb = 0, leaf = 0;
+  double ms = 0.0;
+};
+static std::mutex g_tune_mu;
+static std::map<int, TuneEntry> g_tune;  // bucket -> entry
+static bool g_tune_file_read = false;
+
+int tune_bucket(int n) {
+  int b = 256;
+  while (b < n && b < (1 << 30)) b *= 2;
+  return b;
+}
+static int load_tuning_locked(const char* path);
+static TuneEntry table_entry(int n) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  if (!g_tune_file_read) {
+    g_tune_file_read = true;
+    const char* f = std::getenv("LAPACK_AMD_TUNE_FILE");
+    if (f && *f) (void)load_tuning_locked(f);
+  }
+  auto it = g_tune.find(tune_bucket(n));
+  return it == g_tune.end() ? TuneEntry{} : it->second;
+}
+static thread_local int t_force_nb = 0, t_force_leaf = 0;  // a tuning trial's candidate
+int tune_nb_for(int n) {
+  if (t_force_nb > 0) return t_force_nb;
   int v = g_nb.load();
-  if (v <= 0) v = env_int("LAPACK_AMD_NB", 256);
+  if (v <= 0) v = env_int("LAPACK_AMD_NB", 0);
+  if (v <= 0 && n > 0) v = table_entry(n).nb;
+  if (v <= 0) v = 256;
   return std::max(1, std::min(v, 256));  // the triangle-inverse kernel covers <= 256
 }
-int tune_leaf() {
+int tune_leaf_for(int n) {
+  if (t_force_leaf > 0) return t_force_leaf;
   int v = g_leaf.load();
-  if (v <= 0) v = env_int("LAPACK_AMD_LEAF", 64);
+  if (v <= 0) v = env_int("LAPACK_AMD_LEAF", 0);
+  if (v <= 0 && n > 0) v = table_entry(n).leaf;
+  if (v <= 0) v = 64;
   return std::max(1, std::min(v, 64));
+}
+int tune_nb() { return tune_nb_for(0); }
+int tune_leaf() { return tune_leaf_for(0); }
+
+// The table as JSON: {"getrf": {"<bucket>": {"nb": .., "leaf": .., "ms": ..}, ..}}
+static int load_tuning_locked(const char* path) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) return -1;
+  std::string txt;
+  char buf[4096];
+  size_t k;
+  while ((k = std::fread(buf, 1, sizeof buf, f)) > 0) txt.append(buf, k);
+  std::fclose(f);
+  // every "<bucket>": {"nb": .., "leaf": .., "ms": ..} after the "getrf" key
+  size_t pos = txt.find("\"getrf\"");
+  int count = 0;
+  while (pos != std::string::npos) {
+    const size_t q = txt.find('"', pos + 1 + (count == 0 ? 6 : 0));
+    if (q == std::string::npos || q + 1 >= txt.size() || !std::isdigit((unsigned char)txt[q + 1])) break;
+    const int bucket = std::atoi(txt.c_str() + q + 1);
+    const size_t open = txt.find('{', q), close = txt.find('}', q);
+    if (open == std::string::npos || close == std::string::npos || open > close) break;
+    const std::string body = txt.substr(open, close - open);
+    auto field = [&](const char* key, double dflt) {
+      const size_t k = body.find(key);
+      return k == std::string::npos ? dflt : std::atof(body.c_str() + k + std::strlen(key));
+    };
+    TuneEntry e;
+    e.nb = (int)field("\"nb\":", 0);
+    e.leaf = (int)field("\"leaf\":", 0);
+    e.ms = field("\"ms\":", 0.0);
+    if (bucket > 0 && e.nb > 0 && e.leaf > 0) {
+      g_tune[bucket] = e;
+      ++count;
+    }
+    pos = close;
+  }
+  return count;
+}
+static int save_tuning_locked(const char* path) {
+  FILE* f = std::fopen(path, "w");
+  if (!f) return -1;
+  std::fprintf(f, "{\"getrf\": {");
+  bool first = true;
+  for (auto& kv : g_tune) {
+    std::fprintf(f, "%s\"%d\": {\"nb\": %d, \"leaf\": %d, \"ms\": %.4f}", first ? "" : ", ", kv.first,
+                 kv.second.nb, kv.second.leaf, kv.second.ms);
+    first = false;
+  }
+  std::fprintf(f, "}}\n");
+  std::fclose(f);
+  return 0;
+}
+int load_tuning(const char* path) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune_file_read = true;
+  if (!path) {
+    g_tune.clear();
+    return 0;
+  }
+  return load_tuning_locked(path);
 }
 
 static int hip_check(hipError_t e) {
@@ -100,6 +211,18 @@ static DevCtx* get_ctx() {
     c->dev = dev;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return nullptr; }
     if (hipEventCreateWithFlags(&c->ws_free, hipEventDisableTiming) != hipSuccess) { delete c; return nullptr; }
+    if (hipMalloc(&c->status, 256) != hipSuccess || hipMemset(c->status, 0, 256) != hipSuccess) {
+      (void)hipGetLastError();
+      delete c;
+      return nullptr;
+    }
+    int nxcc = 0, ncu = 0;
+    if (hipDeviceGetAttribute(&nxcc, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      (void)hipGetLastError();
+      nxcc = 0;
+    }
+    c->xcd8 = nxcc == 8 && ncu >= 8 * 32;
     g_ctx[dev] = c;
   }
   return g_ctx[dev];
@@ -188,26 +311,29 @@ template <> struct K<double> {
                      int nskip, hipStream_t s) {
     launch_dgemm_update(m, n, k, A, lda, B, ldb, C, ldc, nskip, s);
   }
-  static void transpose(int r, int c, const double* in, long li, double* out, long lo, hipStream_t s) {
-    launch_transpose_d(r, c, in, li, out, lo, s);
+  static void transpose(int r, int c, const double* in, long li, double* out, long lo, hipStream_t s,
+                        const int* skip = nullptr) {
+    launch_transpose_d(r, c, in, li, out, lo, s, skip);
   }
   static void apply_cols(double* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs, const int* np,
                          hipStream_t s, int nskip = 0) {
     launch_apply_pairs_cols_d(T, ldt, r0, r1, r2, r3, pairs, np, s, nskip);
   }
-  static void gather(int nr, int nc, const int* perm, const double* src, long ls, double* dst, long ld, hipStream_t s) {
-    launch_gather_rows_d(nr, nc, perm, src, ls, dst, ld, s);
+  static void gather(int nr, int nc, const int* perm, const double* src, long ls, double* dst, long ld, hipStream_t s,
+                     const int* skip = nullptr) {
+    launch_gather_rows_d(nr, nc, perm, src, ls, dst, ld, s, skip);
   }
   static void trtri_blocks(bool up, bool unit, int n, int nb, const double* U, long ldu, double* W, long ldw,
                            hipStream_t s) {
     launch_trtri_blocks_d(up, unit, n, nb, U, ldu, W, ldw, s);
   }
-  static void copy(int r, int c, const double* src, long ls, double* dst, long ld, hipStream_t s, int nskip = 0) {
-    launch_copy_d(r, c, src, ls, dst, ld, s, nskip);
+  static void copy(int r, int c, const double* src, long ls, double* dst, long ld, hipStream_t s, int nskip = 0,
+                   const int* skip = nullptr) {
+    launch_copy_d(r, c, src, ls, dst, ld, s, nskip, skip);
   }
   static int panel(double* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned tagbase, double* x,
-                   unsigned* tags, int* err, unsigned* rk, bool xcd0, hipStream_t s) {
-    return launch_getf2_panel_d(T, ldt, j0, w, m, ipiv, info, tagbase, x, tags, err, rk, xcd0, s);
+                   int* err, int mode, hipStream_t s) {
+    return launch_getf2_panel_d(T, ldt, j0, w, m, ipiv, info, tagbase, x, err, mode, s);
   }
   static void boundary(double* T, long ldt, int j, int jj, int w, int jend, int m, const int* ipiv, hipStream_t s) {
     launch_panel_boundary_d(T, ldt, j, jj, w, jend, m, ipiv, s);
@@ -230,26 +356,29 @@ template <> struct K<float> {
     else
       launch_sgemm_update(m, n, k, A, lda, B, ldb, C, ldc, nskip, s);
   }
-  static void transpose(int r, int c, const float* in, long li, float* out, long lo, hipStream_t s) {
-    launch_transpose_s(r, c, in, li, out, lo, s);
+  static void transpose(int r, int c, const float* in, long li, float* out, long lo, hipStream_t s,
+                        const int* skip = nullptr) {
+    launch_transpose_s(r, c, in, li, out, lo, s, skip);
   }
   static void apply_cols(float* T, long ldt, int r0, int r1, int r2, int r3, const int* pairs, const int* np,
                          hipStream_t s, int nskip = 0) {
     launch_apply_pairs_cols_s(T, ldt, r0, r1, r2, r3, pairs, np, s, nskip);
   }
-  static void gather(int nr, int nc, const int* perm, const float* src, long ls, float* dst, long ld, hipStream_t s) {
-    launch_gather_rows_s(nr, nc, perm, src, ls, dst, ld, s);
+  static void gather(int nr, int nc, const int* perm, const float* src, long ls, float* dst, long ld, hipStream_t s,
+                     const int* skip = nullptr) {
+    launch_gather_rows_s(nr, nc, perm, src, ls, dst, ld, s, skip);
   }
   static void trtri_blocks(bool up, bool unit, int n, int nb, const float* U, long ldu, float* W, long ldw,
                            hipStream_t s) {
     launch_trtri_blocks_s(up, unit, n, nb, U, ldu, W, ldw, s);
   }
-  static void copy(int r, int c, const float* src, long ls, float* dst, long ld, hipStream_t s, int nskip = 0) {
-    launch_copy_s(r, c, src, ls, dst, ld, s, nskip);
+  static void copy(int r, int c, const float* src, long ls, float* dst, long ld, hipStream_t s, int nskip = 0,
+                   const int* skip = nullptr) {
+    launch_copy_s(r, c, src, ls, dst, ld, s, nskip, skip);
   }
   static int panel(float* T, long ldt, int j0, int w, int m, int* ipiv, int* info, unsigned tagbase, double* x,
-                   unsigned* tags, int* err, unsigned* rk, bool xcd0, hipStream_t s) {
-    return launch_getf2_panel_s(T, ldt, j0, w, m, ipiv, info, tagbase, x, tags, err, rk, xcd0, s);
+                   int* err, int mode, hipStream_t s) {
+    return launch_getf2_panel_s(T, ldt, j0, w, m, ipiv, info, tagbase, x, err, mode, s);
   }
   static void boundary(float* T, long ldt, int j, int jj, int w, int jend, int m, const int* ipiv, hipStream_t s) {
     launch_panel_boundary_s(T, ldt, j, jj, w, jend, m, ipiv, s);
@@ -337,11 +466,9 @@ size_t getrf_ws_bytes(int m, int n, int nb, int leaf) {
     cv.take<T>((size_t)nb * nb);
     cv.take<int>((size_t)2 * swp_max_pairs() + 64);
   }
-  cv.take<unsigned>(panel_tag_count());           // panel exchange tags
-  cv.take<char>(panel_xchg_bytes(0));
+  cv.take<char>(panel_xchg_bytes(0));             // panel exchange + tall-leaf scratch
   cv.take<int>(4);                                // err flag
   cv.take<int>((size_t)m);                        // position map (fallback)
-  cv.take<unsigned>((size_t)std::min(m, n) + 8);  // per-leaf rank counters
   return cv.off;
 }
 
@@ -354,42 +481,23 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
   DevCtx* c = get_ctx();
   if (!c) return LA_ERR_NODEVICE;
   std::lock_guard<std::mutex> g(c->mu);
-  if (m == 0 || n == 0) return 0;
-  if (m > 32768) return LA_ERR_LIMIT;  // panel kernel: <= 64 workgroups x 512 rows
-  const int nb = tune_nb();
+  if (m == 0 || n == 0) {
+    launch_fill_i32(info, 1, 0, s);  // INFO = 0 as the reference's quick return
+    return 0;
+  }
   const int mn = std::min(m, n);
+  const int nb = tune_nb_for(mn);
+  const int leaf = tune_leaf_for(mn);
   const long ldt = (long)align_up((size_t)n, 32);
   const long ldx = ldt;
   char* ws = nullptr;
-  int rc = ws_acquire(c, getrf_ws_bytes<T>(m, n, nb, tune_leaf()), s, &ws);
+  int rc = ws_acquire(c, getrf_ws_bytes<T>(m, n, nb, leaf), s, &ws);
   if (rc) return rc;
   if (!c->pstream) {
-    // The panel chain is latency-bound and runs beside the trailing update:
-    // give it CUs of its own (LAPACK_AMD_PANEL_CUS per device, 0 = share all).
-    int ncu = 0;
-    LA_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
-    const int pcu = std::min(std::max(env_int("LAPACK_AMD_PANEL_CUS", 0), 0), ncu / 2);
-    if (pcu > 0 && ncu <= 1024) {
-      std::vector<uint32_t> mp((ncu + 31) / 32, 0u), mu((ncu + 31) / 32, 0u);
-      // spread the panel's CUs evenly over the CU index space (all XCDs)
-      const int stride = ncu / pcu;
-      for (int i = 0; i < ncu; ++i) {
-        const bool panel = (i % stride == 0) && (i / stride < pcu);
-        (panel ? mp : mu)[i / 32] |= 1u << (i % 32);
-      }
-      if (hipExtStreamCreateWithCUMask(&c->pstream, (uint32_t)mp.size(), mp.data()) != hipSuccess ||
-          hipExtStreamCreateWithCUMask(&c->ustream, (uint32_t)mu.size(), mu.data()) != hipSuccess) {
-        // CU-masked queues refused by the runtime: fall back to plain streams
-        (void)hipGetLastError();
-        if (c->pstream) hipStreamDestroy(c->pstream);
-        c->pstream = nullptr;
-        c->ustream = nullptr;
-      }
-    }
-    if (!c->pstream) {
-      LA_TRY(hipStreamCreateWithFlags(&c->pstream, hipStreamNonBlocking));
-      LA_TRY(hipStreamCreateWithFlags(&c->ustream, hipStreamNonBlocking));
-    }
+    // the panel chain (latency-bound) and the trailing update run on two
+    // streams of their own, fenced to the caller's stream
+    LA_TRY(hipStreamCreateWithFlags(&c->pstream, hipStreamNonBlocking));
+    LA_TRY(hipStreamCreateWithFlags(&c->ustream, hipStreamNonBlocking));
     for (int i = 0; i < 5; ++i) LA_TRY(hipEventCreateWithFlags(&c->ev[i], hipEventDisableTiming));
   }
   hipStream_t sp = c->pstream;
@@ -409,14 +517,16 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
     Ws[i] = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nb * nb));
     Ps[i] = reinterpret_cast<int*>(ws + cv.take<int>((size_t)2 * swp_max_pairs() + 64));
   }
-  unsigned* tags = reinterpret_cast<unsigned*>(ws + cv.take<unsigned>(panel_tag_count()));
   double* xchg = reinterpret_cast<double*>(ws + cv.take<char>(panel_xchg_bytes(0)));
   int* err = reinterpret_cast<int*>(ws + cv.take<int>(4));
   int* gmap = reinterpret_cast<int*>(ws + cv.take<int>((size_t)m));
-  const size_t nrank = (size_t)mn + 8;
-  unsigned* rankctr = reinterpret_cast<unsigned*>(ws + cv.take<unsigned>(nrank));
   const long ldw = nb;
-  const bool xcdsplit = env_int("LAPACK_AMD_PANEL_XCD", 1) != 0;
+  // the multi-workgroup panel owns XCD 0 only on a device of 8 XCDs; a
+  // redone call (t_safe) takes the panel form without waits
+  const bool xcdsplit = c->xcd8 && env_int("LAPACK_AMD_PANEL_XCD", 1) != 0 && !t_safe;
+  int fail = g_fail_waits.load();
+  while (fail > 0 && !g_fail_waits.compare_exchange_weak(fail, fail - 1)) {
+  }
   // panel workgroups from which the rest's interchanges and DTRSM go first
   const int pre_pg = env_int("LAPACK_AMD_PRE_PG", 27);
 
@@ -434,11 +544,9 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
   auto endm = [&](PhaseTimer& pt, std::vector<double>& fl) { mark(pt, fl, -1, 0); };
   auto Tp = [&](long r, long col) { return Tm + r + col * ldt; };
 
-  launch_fill_u32(tags, panel_tag_count(), 0u, s);
   // the panel's flag-in-data exchange needs a zeroed buffer per factorization
   launch_fill_u32(reinterpret_cast<unsigned*>(xchg), panel_xchg_bytes(0) / sizeof(unsigned), 0u, s);
-  launch_fill_u32(rankctr, nrank, 0u, s);
-  launch_fill_i32(err, 1, 0, s);
+  launch_fill_i32(err, 1, fail > 0 ? 1 : 0, s);
   launch_fill_i32(info, 1, 0, s);
   mark(pa, fla, LA_PHASE_TRANSPOSE, 0);
   K<T>::transpose(m, n, A, lda, Tm, ldt, s);  // T(c, r) = A(r, c)
@@ -450,13 +558,13 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
   // DGEMM between leaves (buffers of slot 1, stream st).
   auto factor_panel = [&](int j, int jb, hipStream_t st, PhaseTimer& pt, std::vector<double>& fl,
                           bool xcd0) -> int {
+    const int mode = t_safe ? PANEL_TALL : xcd0 ? PANEL_XCD0 : PANEL_PLAIN;
     for (int jj = j; jj < j + jb;) {
       const int h = m - jj;
-      const int w = std::min({tune_leaf(), panel_max_width(h), j + jb - jj});
+      const int w = std::min({leaf, panel_max_width(h), j + jb - jj});
       mark(pt, fl, LA_PHASE_PANEL, 0);
       ++leafno;
-      int r = K<T>::panel(Tm, ldt, jj, w, m, ipiv, info, 64u * (unsigned)leafno, xchg, tags, err,
-                          rankctr + (leafno % (int)nrank), xcd0, st);
+      int r = K<T>::panel(Tm, ldt, jj, w, m, ipiv, info, 64u * (unsigned)leafno, xchg, err, mode, st);
       endm(pt, fl);
       if (r) return r;
       const int rest = j + jb - jj - w;
@@ -586,10 +694,13 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
     ws_release(c, s_user);
     return rc;
   }
+  // the factors go back to A only when no bounded wait gave up: otherwise A
+  // is left as the caller passed it, INFO = LA_ERR_SYNC, and the call can be
+  // redone
   mark(pa, fla, LA_PHASE_TRANSPOSE, 0);
-  K<T>::transpose(n, m, Tm, ldt, A, lda, s);
+  K<T>::transpose(n, m, Tm, ldt, A, lda, s, err);
   endm(pa, fla);
-  launch_fold_err(err, info, s);
+  launch_fold_err(err, info, c->status, s);
   LA_TRY(hipEventRecord(c->ev[3], s));
   LA_TRY(hipStreamWaitEvent(s_user, c->ev[3], 0));
   rc = ws_release(c, s_user);
@@ -614,23 +725,18 @@ size_t getrs_ws_bytes(int n, int nrhs, int nbs) {
   return cv.off;
 }
 
-// The blocked-GEMM form of the sweeps (diagonal-block inverses + MFMA
-// products) is kept for comparison: LAPACK_AMD_GETRS=gemm selects it.
-static bool getrs_use_gemm() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = std::getenv("LAPACK_AMD_GETRS");
-    v = (e && std::strcmp(e, "gemm") == 0) ? 1 : 0;
-  }
-  return v != 0;
-}
-
 // Solve op(A) X = B with the LU of getrf (SRC/dgetrs.f:181-218).  Device
-// pointers, asynchronous on s.  Blocked substitution: the diagonal blocks
-// of L and U are inverted once, every block step is then two MFMA GEMMs.
+// pointers, asynchronous on s.  Two forms of the triangular sweeps:
+//   * one tri_chain launch per sweep (getrs.hip), the default: workgroups in
+//     ticket order with bounded waits for the blocks they depend on.  When a
+//     wait gives up, B is left as passed, bit 0 of the device's sticky
+//     status is set and, when waitflag is given, so is *waitflag (a device
+//     int), so that a synchronous caller can redo the solve;
+//   * the blocked form without waits (t_safe, a redone call): the diagonal
+//     blocks of L and U inverted once, every block step two MFMA GEMMs.
 template <class T>
-int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* ipiv, T* B, long ldb,
-                 hipStream_t s) {
+int getrs_device_ex(char trans, int n, int nrhs, const T* A, long lda, const int* ipiv, T* B, long ldb,
+                    hipStream_t s, int* waitflag) {
   DevCtx* c = get_ctx();
   if (!c) return LA_ERR_NODEVICE;
   std::lock_guard<std::mutex> g(c->mu);
@@ -660,21 +766,25 @@ int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* i
   std::vector<double> fl;
   if (pt.mark(LA_PHASE_GETRS)) fl.push_back(2.0 * (double)n * n * nrhs);
 
-  if (!getrs_use_gemm()) {
-    launch_fill_i32(cerr, 1, 0, s);
+  if (!t_safe) {
+    int fail = g_fail_waits.load();
+    while (fail > 0 && !g_fail_waits.compare_exchange_weak(fail, fail - 1)) {
+    }
+    launch_fill_i32(cerr, 1, fail > 0 ? 1 : 0, s);
     if (notran) {
       launch_swaps_to_perm(n, 0, n, ipiv, 1, perm, s, gpairs);
       K<T>::gather(n, nrhs, perm, B, ldb, X, ldx, s);                // X = P*B (dgetrs.f:187)
       launch_tri_chain<T>(0, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // L Y = X (dgetrs.f:191)
       launch_tri_chain<T>(1, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // U X = Y (dgetrs.f:197)
-      K<T>::copy(n, nrhs, X, ldx, B, ldb, s);
+      K<T>::copy(n, nrhs, X, ldx, B, ldb, s, 0, cerr);
     } else {
       K<T>::copy(n, nrhs, B, ldb, X, ldx, s);
       launch_tri_chain<T>(2, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // U^T Y = B (dgetrs.f:205)
       launch_tri_chain<T>(3, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // L^T X = Y (dgetrs.f:211)
       launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s, gpairs);               // dgetrs.f:217
-      K<T>::gather(n, nrhs, perm, X, ldx, B, ldb, s);
+      K<T>::gather(n, nrhs, perm, X, ldx, B, ldb, s, cerr);
     }
+    launch_fold_err(cerr, waitflag, c->status, s);
     if (pt.mark(-1)) fl.push_back(0);
     rc = ws_release(c, s);
     phase_collect(pt, fl);
@@ -727,6 +837,12 @@ int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* i
   return rc;
 }
 
+template <class T>
+int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* ipiv, T* B, long ldb,
+                 hipStream_t s) {
+  return getrs_device_ex<T>(trans, n, nrhs, A, lda, ipiv, B, ldb, s, nullptr);
+}
+
 // DLASWP on a column-major device matrix (SRC/dlaswp.f:112-189).  nrows is
 // the number of rows the interchanges may reach.
 template <class T>
@@ -767,10 +883,109 @@ template int getrf_device<double>(int, int, double*, long, int*, int*, hipStream
 template int getrf_device<float>(int, int, float*, long, int*, int*, hipStream_t);
 template int getrs_device<double>(char, int, int, const double*, long, const int*, double*, long, hipStream_t);
 template int getrs_device<float>(char, int, int, const float*, long, const int*, float*, long, hipStream_t);
+template int getrs_device_ex<double>(char, int, int, const double*, long, const int*, double*, long, hipStream_t,
+                                     int*);
+template int getrs_device_ex<float>(char, int, int, const float*, long, const int*, float*, long, hipStream_t,
+                                    int*);
 template int laswp_device<double>(int, double*, long, int, int, int, const int*, int, hipStream_t);
 template int laswp_device<float>(int, float*, long, int, int, int, const int*, int, hipStream_t);
 
 int last_error() { return t_last_error; }
+
+// Autotuner of the plain factorization: times DGETRF of an n x n matrix
+// (uniform(-1,1) entries from a device generator) for each candidate
+// (NB, leaf) -- one untimed call, then the best of two timed ones, HIP
+// events on the library's stream -- and keeps the fastest for n's bucket
+// (the next power of two); with path, the table is written there as JSON
+// (and read back by lapack_amd_load_tuning or LAPACK_AMD_TUNE_FILE).
+int tune_getrf(int n, const char* path, int* nb_out, int* leaf_out, double* ms_out) {
+  if (n <= 0) return -1;
+  DevCtx* c = get_ctx();
+  if (!c) return LA_ERR_NODEVICE;
+  const size_t bytes = sizeof(double) * (size_t)n * n;
+  double *A0 = nullptr, *A = nullptr;
+  int *ipiv = nullptr, *info = nullptr;
+  if (hipMalloc(&A0, bytes) != hipSuccess || hipMalloc(&A, bytes) != hipSuccess ||
+      hipMalloc(&ipiv, sizeof(int) * (size_t)n + 256) != hipSuccess) {
+    (void)hipGetLastError();
+    if (A0) hipFree(A0);
+    if (A) hipFree(A);
+    return LA_ERR_NOMEM;
+  }
+  info = ipiv + n;
+  hipStream_t s = c->stream;
+  launch_fill_random(A0, (size_t)n * n, 0x9e3779b97f4a7c15ull ^ (unsigned long long)n, s);
+  static const int cand[][2] = {{256, 64}, {192, 64}, {128, 64}, {256, 32}, {192, 32}, {128, 32}};
+  hipEvent_t e0, e1;
+  int rc = 0;
+  TuneEntry best;
+  best.ms = 1e300;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = LA_ERR_HIP;
+  for (const auto& cd : cand) {
+    if (rc) break;
+    t_force_nb = cd[0];
+    t_force_leaf = cd[1];
+    double tmin = 1e300;
+    for (int rep = 0; rep < 3 && !rc; ++rep) {
+      if (hipMemcpyAsync(A, A0, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) { rc = LA_ERR_HIP; break; }
+      hipEventRecord(e0, s);
+      rc = getrf_device<double>(n, n, A, n, ipiv, info, s);
+      hipEventRecord(e1, s);
+      if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = LA_ERR_HIP;
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (rep > 0) tmin = std::min(tmin, (double)ms);
+    }
+    int h_info = 0;
+    if (!rc) hipMemcpy(&h_info, info, sizeof(int), hipMemcpyDeviceToHost);
+    if (!rc && h_info == 0 && tmin < best.ms) {
+      best.nb = cd[0];
+      best.leaf = cd[1];
+      best.ms = tmin;
+    }
+  }
+  t_force_nb = 0;
+  t_force_leaf = 0;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(A0);
+  hipFree(A);
+  hipFree(ipiv);
+  if (rc) return rc;
+  if (best.nb <= 0) return LA_ERR_SYNC;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune_file_read = true;
+    if (path && *path) (void)load_tuning_locked(path);  // keep the other buckets
+    g_tune[tune_bucket(n)] = best;
+    if (path && *path && save_tuning_locked(path) != 0) return LA_ERR_HIP;
+  }
+  if (nb_out) *nb_out = best.nb;
+  if (leaf_out) *leaf_out = best.leaf;
+  if (ms_out) *ms_out = best.ms;
+  return 0;
+}
+
+// Runs f() with the forms without waits selected (a redone call).
+int with_safe_forms(const std::function<int()>& f) {
+  const int old = t_safe;
+  t_safe = 1;
+  const int rc = f();
+  t_safe = old;
+  return rc;
+}
+void fail_waits(int count) { g_fail_waits = count > 0 ? count : 0; }
+// Bit 0: a bounded wait of a kernel launched on the current device gave up
+// since the last clear.  Waits for the device to be idle first.
+int device_status(int clear) {
+  DevCtx* c = get_ctx();
+  if (!c) return LA_ERR_NODEVICE;
+  LA_TRY(hipDeviceSynchronize());
+  int v = 0;
+  LA_TRY(hipMemcpy(&v, c->status, sizeof(int), hipMemcpyDeviceToHost));
+  if (clear && v) LA_TRY(hipMemset(c->status, 0, sizeof(int)));
+  return v;
+}
 void set_tuning(int nb, int leaf) { g_nb = nb; g_leaf = leaf; }
 DevCtx* ctx_for_current() { return get_ctx(); }
 hipStream_t ctx_stream(DevCtx* c) { return c->stream; }

@@ -15,7 +15,9 @@
 // the comparisons are made with the reference's exact expressions.
 #include "la_common.h"
 
+#include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -29,6 +31,7 @@ template <class T>
 int getrs_device(char trans, int n, int nrhs, const T* A, long lda, const int* ipiv, T* B, long ldb, hipStream_t s);
 DevCtx* ctx_for_current();
 int mp_buffer(DevCtx* c, size_t bytes, char** out);
+int with_safe_forms(const std::function<int()>& f);
 std::mutex& mp_mutex(DevCtx* c);
 
 namespace {
@@ -281,6 +284,7 @@ struct IrBufs {
   double* sc;  // scalars
   double* gws; // gemv workspace
   double* LU64; // FP64 copy of the low-precision factors (GMRES's preconditioner)
+  double* Minv; // C (LU)^-1 R formed in FP64 from LU64 (n <= 16384), or null
   unsigned long long* cm;
   int* flags;
 };
@@ -311,15 +315,20 @@ int dgesv_ir_device(int n, int nrhs, double* A, long lda, int* ipiv, const doubl
   DevCtx* ctx = ctx_for_current();
   if (!ctx) return LA_ERR_NODEVICE;
   std::lock_guard<std::mutex> g(mp_mutex(ctx));
-  const int GM = 50;  // GMRES restart length
+  // GMRES basis: up to min(n, 8192) vectors (restart length), within about
+  // 2 GiB; the explicit inverse of the low-precision LU (GMRES's
+  // preconditioner, formed once in FP64) for n <= 16384
+  const int GM = (int)std::max<long>(50, std::min<long>({(long)n, 8192L, (2L << 30) / (8L * n)}));
+  const bool minv_form = n <= 16384;
   const int nr = std::max(nrhs, 1);
   const int rch = std::min(nr, 16);
   const size_t gws_n = gemv_ws_doubles(n, std::max(n, GM + 1), rch);
   const size_t need = al256(sizeof(float) * (size_t)n * n) + al256(sizeof(double) * (size_t)n * n) +
+                      (minv_form ? al256(sizeof(double) * (size_t)n * n) : 0) +
                       al256(sizeof(float) * (size_t)n * nr) +
                       al256(sizeof(double) * (size_t)n * nr) + 2 * al256(sizeof(double) * (size_t)n) +
                       al256(sizeof(double) * (size_t)n * (GM + 1)) + 2 * al256(sizeof(double) * (size_t)n) +
-                      al256(sizeof(double) * 2 * (GM + 2)) + al256(sizeof(double) * 64) +
+                      al256(sizeof(double) * (2 * (size_t)GM + 8)) + al256(sizeof(double) * 64) +
                       al256(sizeof(double) * gws_n) +
                       al256(sizeof(unsigned long long) * 2 * (size_t)nr) + 256;
   char* buf = nullptr;
@@ -330,6 +339,7 @@ int dgesv_ir_device(int n, int nrhs, double* A, long lda, int* ipiv, const doubl
   IrBufs b;
   b.SA = reinterpret_cast<float*>(take(sizeof(float) * (size_t)n * n));
   b.LU64 = reinterpret_cast<double*>(take(sizeof(double) * (size_t)n * n));
+  b.Minv = minv_form ? reinterpret_cast<double*>(take(sizeof(double) * (size_t)n * n)) : nullptr;
   b.SX = reinterpret_cast<float*>(take(sizeof(float) * (size_t)n * nr));
   b.R = reinterpret_cast<double*>(take(sizeof(double) * (size_t)n * nr));
   b.rr = reinterpret_cast<double*>(take(sizeof(double) * (size_t)n));
@@ -337,7 +347,7 @@ int dgesv_ir_device(int n, int nrhs, double* A, long lda, int* ipiv, const doubl
   b.V = reinterpret_cast<double*>(take(sizeof(double) * (size_t)n * (GM + 1)));
   b.t = reinterpret_cast<double*>(take(sizeof(double) * (size_t)n));
   b.w = reinterpret_cast<double*>(take(sizeof(double) * (size_t)n));
-  b.h = reinterpret_cast<double*>(take(sizeof(double) * 2 * (GM + 2)));
+  b.h = reinterpret_cast<double*>(take(sizeof(double) * (2 * (size_t)GM + 8)));
   b.sc = reinterpret_cast<double*>(take(sizeof(double) * 64));
   b.gws = reinterpret_cast<double*>(take(sizeof(double) * gws_n));
   b.cm = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * 2 * nr));
@@ -383,12 +393,26 @@ int dgesv_ir_device(int n, int nrhs, double* A, long lda, int* ipiv, const doubl
     return 0;
   };
   // GMRES's preconditioner, applied in FP64 with the low-precision factors
-  // (the working-precision application of GMRES-IR): y <- C (LU)^-1 R v
+  // (the working-precision application of GMRES-IR): y <- C (LU)^-1 R v.
+  // For n <= 16384 the operator is formed once, explicitly: (LU)^-1 by the
+  // blocked triangular solves on the identity (no waits, MFMA products),
+  // then scaled by C and R, so each application is one pass over an n x n
+  // matrix instead of two latency-bound sweeps.
   bool lu64_ready = false;
   auto precond64 = [&](const double* v, double* y) -> int {
     if (!lu64_ready) {
       launch_cast_s2d(n, n, b.SA, ln, b.LU64, ln, s);
+      if (b.Minv) {
+        launch_identity(n, b.Minv, ln, s);
+        int r3 = with_safe_forms([&]() { return getrs_device<double>('N', n, n, b.LU64, ln, ipiv, b.Minv, ln, s); });
+        if (r3) return r3;
+        if (rr || cc) launch_scale_rc(n, n, b.Minv, ln, cc, rr, cc && rr ? 3 : cc ? 1 : 2, s);
+      }
       lu64_ready = true;
+    }
+    if (b.Minv) {
+      launch_dgemv_n(n, n, 1, 1.0, b.Minv, ln, v, ln, 0.0, y, ln, b.gws, s);
+      return 0;
     }
     if (v != y) launch_copy_d(n, 1, v, ln, y, ln, s);
     if (rr) launch_scale_rc(n, 1, y, ln, rr, nullptr, 1, s);
@@ -459,25 +483,33 @@ int dgesv_ir_device(int n, int nrhs, double* A, long lda, int* ipiv, const doubl
       residual();
       if ((rc = check(&all))) return rc;
     }
-    // GMRES-IR on the columns still unconverged
+    // GMRES-IR on the columns still unconverged: GMRES(GM) on A d = r,
+    // left-preconditioned, classical Gram-Schmidt applied twice in FP64,
+    // Givens rotations and the small triangular solve on the host.  At most
+    // 30 restarts and 2n inner iterations in all.
     int cycles = 0;
-    while (!all && cycles < max_cycles) {
+    const long max_inner = 2L * n;
+    // Hessenberg columns after rotation (column j holds j + 1 entries at
+    // offset j (j + 1) / 2), Givens cosines / sines, the rotated residual
+    std::vector<double> H, cs(GM, 0.0), sn(GM, 0.0), gv(GM + 1, 0.0), hh(2 * (size_t)GM + 8);
+    while (!all && cycles < max_cycles && *gmres_iter < max_inner) {
       for (int col = 0; col < nrhs; ++col) {
         if (done[col]) continue;
         double* xc = X + (long)col * ldx;
         // z = M^-1 r, beta = ||z||_2, V0 = z / beta
         if ((rc = precond64(b.R + (long)col * ln, b.w))) return rc;
-        launch_dgemv_t(n, 1, 1.0, b.w, ln, b.w, 0.0, b.sc, b.gws, s);
+        launch_dgemv_cols(n, 1, 1.0, b.w, ln, b.w, 0.0, b.sc, false, s);
         double nrm2 = 0.0;
         MP_CK(hipMemcpyAsync(&nrm2, b.sc, sizeof(double), hipMemcpyDeviceToHost, s));
         MP_CK(hipStreamSynchronize(s));
         const double beta = std::sqrt(nrm2);
         if (!(beta > 0.0) || !std::isfinite(beta)) continue;
-        std::vector<double> H((size_t)(GM + 1) * GM, 0.0), cs(GM, 0.0), sn(GM, 0.0), gv(GM + 1, 0.0);
+        H.assign((size_t)GM * (GM + 1) / 2 + GM, 0.0);
+        std::fill(gv.begin(), gv.end(), 0.0);
         gv[0] = beta;
         {
-          std::vector<double> one(1, 1.0 / beta);
-          MP_CK(hipMemcpyAsync(b.sc + 1, one.data(), sizeof(double), hipMemcpyHostToDevice, s));
+          const double inv = 1.0 / beta;
+          MP_CK(hipMemcpyAsync(b.sc + 1, &inv, sizeof(double), hipMemcpyHostToDevice, s));
           // V0 = w * (1/beta) via the product of a 1-column matrix with a 1-vector
           launch_dgemv_n(n, 1, 1, 1.0, b.w, ln, b.sc + 1, 1, 0.0, b.V, ln, b.gws, s);
         }
@@ -485,53 +517,57 @@ int dgesv_ir_device(int n, int nrhs, double* A, long lda, int* ipiv, const doubl
         const double tol = 1e-12;
         for (; k < GM; ++k) {
           double* vk = b.V + (long)k * ln;
+          const int kk = k + 1;  // basis vectors so far
           launch_dgemv_n(n, n, 1, 1.0, A, lda, vk, ln, 0.0, b.t, ln, b.gws, s);   // t = A v_k
           if ((rc = precond64(b.t, b.w))) return rc;                              // w = M^-1 t
-          launch_dgemv_t(n, k + 1, 1.0, b.V, ln, b.w, 0.0, b.h, b.gws, s);        // h1 = V^T w
-          launch_dgemv_n(n, k + 1, 1, -1.0, b.V, ln, b.h, k + 1, 1.0, b.w, ln, b.gws, s);
-          launch_dgemv_t(n, k + 1, 1.0, b.V, ln, b.w, 0.0, b.h + GM + 2, b.gws, s);  // h2
-          launch_dgemv_n(n, k + 1, 1, -1.0, b.V, ln, b.h + GM + 2, k + 1, 1.0, b.w, ln, b.gws, s);
-          launch_dgemv_t(n, 1, 1.0, b.w, ln, b.w, 0.0, b.sc, b.gws, s);
-          std::vector<double> hh(2 * (GM + 2));
-          MP_CK(hipMemcpyAsync(hh.data(), b.h, sizeof(double) * 2 * (GM + 2), hipMemcpyDeviceToHost, s));
-          MP_CK(hipMemcpyAsync(&nrm2, b.sc, sizeof(double), hipMemcpyDeviceToHost, s));
+          // two passes of classical Gram-Schmidt: h1 = V^T w, w -= V h1;
+          // h2 = V^T w, w -= V h2; then ||w||^2 -- one copy to the host
+          launch_dgemv_cols(n, kk, 1.0, b.V, ln, b.w, 0.0, b.h, false, s);
+          launch_dgemv_n(n, kk, 1, -1.0, b.V, ln, b.h, kk, 1.0, b.w, ln, b.gws, s);
+          launch_dgemv_cols(n, kk, 1.0, b.V, ln, b.w, 0.0, b.h + kk, false, s);
+          launch_dgemv_n(n, kk, 1, -1.0, b.V, ln, b.h + kk, kk, 1.0, b.w, ln, b.gws, s);
+          launch_dgemv_cols(n, 1, 1.0, b.w, ln, b.w, 0.0, b.h + 2 * kk, false, s);
+          MP_CK(hipMemcpyAsync(hh.data(), b.h, sizeof(double) * (2 * (size_t)kk + 1), hipMemcpyDeviceToHost, s));
           MP_CK(hipStreamSynchronize(s));
           ++*gmres_iter;
-          double* hc = &H[(size_t)k * (GM + 1)];
-          for (int i = 0; i <= k; ++i) hc[i] = hh[i] + hh[GM + 2 + i];
-          const double hn1 = std::sqrt(nrm2);
-          hc[k + 1] = hn1;
+          double* hc = &H[(size_t)k * (k + 1) / 2];
+          std::vector<double> col_h(kk + 1);
+          for (int i = 0; i < kk; ++i) col_h[i] = hh[i] + hh[kk + i];
+          const double hn1 = std::sqrt(hh[2 * kk]);
+          col_h[kk] = hn1;
           for (int i = 0; i < k; ++i) {  // previous rotations
-            const double t1 = cs[i] * hc[i] + sn[i] * hc[i + 1];
-            hc[i + 1] = -sn[i] * hc[i] + cs[i] * hc[i + 1];
-            hc[i] = t1;
+            const double t1 = cs[i] * col_h[i] + sn[i] * col_h[i + 1];
+            col_h[i + 1] = -sn[i] * col_h[i] + cs[i] * col_h[i + 1];
+            col_h[i] = t1;
           }
-          const double den = std::hypot(hc[k], hc[k + 1]);
-          cs[k] = den > 0 ? hc[k] / den : 1.0;
-          sn[k] = den > 0 ? hc[k + 1] / den : 0.0;
-          hc[k] = den;
-          hc[k + 1] = 0.0;
-          gv[k + 1] = -sn[k] * gv[k];
+          const double den = std::hypot(col_h[k], col_h[kk]);
+          cs[k] = den > 0 ? col_h[k] / den : 1.0;
+          sn[k] = den > 0 ? col_h[kk] / den : 0.0;
+          col_h[k] = den;
+          for (int i = 0; i < kk; ++i) hc[i] = col_h[i];
+          gv[kk] = -sn[k] * gv[k];
           gv[k] = cs[k] * gv[k];
-          const bool stop = std::fabs(gv[k + 1]) <= tol * beta || !(hn1 > 0.0) || k + 1 == GM;
+          const bool stop = std::fabs(gv[kk]) <= tol * beta || !(hn1 > 0.0) || kk == GM ||
+                            *gmres_iter >= max_inner;
           if (!stop) {
-            std::vector<double> inv(1, 1.0 / hn1);
-            MP_CK(hipMemcpyAsync(b.sc + 1, inv.data(), sizeof(double), hipMemcpyHostToDevice, s));
-            launch_dgemv_n(n, 1, 1, 1.0, b.w, ln, b.sc + 1, 1, 0.0, b.V + (long)(k + 1) * ln, ln, b.gws, s);
+            const double inv = 1.0 / hn1;
+            MP_CK(hipMemcpyAsync(b.sc + 1, &inv, sizeof(double), hipMemcpyHostToDevice, s));
+            launch_dgemv_n(n, 1, 1, 1.0, b.w, ln, b.sc + 1, 1, 0.0, b.V + (long)kk * ln, ln, b.gws, s);
           } else {
-            ++k;
+            k = kk;
             break;
           }
         }
-        // y = H(0:k,0:k)^-1 g, x += V y
+        // y = Rk^-1 g (Rk the rotated Hessenberg, upper triangular), x += V y
         std::vector<double> y(k, 0.0);
         for (int i = k - 1; i >= 0; --i) {
           double sacc = gv[i];
-          for (int j = i + 1; j < k; ++j) sacc -= H[(size_t)j * (GM + 1) + i] * y[j];
-          y[i] = sacc / H[(size_t)i * (GM + 1) + i];
+          for (int j = i + 1; j < k; ++j) sacc -= H[(size_t)j * (j + 1) / 2 + i] * y[j];
+          y[i] = sacc / H[(size_t)i * (i + 1) / 2 + i];
         }
         MP_CK(hipMemcpyAsync(b.h, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
         launch_dgemv_n(n, k, 1, 1.0, b.V, ln, b.h, k, 1.0, xc, ldx, b.gws, s);
+        MP_CK(hipStreamSynchronize(s));  // y's host buffer is released below
       }
       ++cycles;
       residual();
@@ -768,6 +804,162 @@ void lacn2_host(int n, double* v, double* x, int* isgn, double* est, int* kase, 
       return;
     }
   }
+}
+
+
+// ============================================================================
+// DGECON: reciprocal condition number estimate from the factors of DGETRF
+// (SRC/dgecon.f:130-244)
+// ============================================================================
+// DLACN2's reverse communication on the host (n-vectors), the triangular
+// solves with L and U (or their transposes) on the GPU, one tri_chain sweep
+// each.  DLATRS's scaling (SRC/dlatrs.f) only departs from a plain solve when
+// the solution would overflow; there the reference finds SCALE < |x|*SMLNUM
+// and returns RCOND = 0 (dgecon.f:226-229), which is what a solve that
+// overflows to a non-finite value returns here.  anorm: the norm of the
+// original matrix, checked by the caller.  Returns INFO (0, or 1 when the
+// estimate of ||inv(A)|| is zero or RCOND is NaN / above the overflow
+// threshold, dgecon.f:233-242) or a negative LA_ERR_* code.
+int gecon_device(bool onenrm, int n, const double* AF, long ldaf, double anorm, double* rcond, hipStream_t s) {
+  *rcond = 0.0;
+  if (n == 0) {
+    *rcond = 1.0;
+    return 0;
+  }
+  if (anorm == 0.0) return 0;
+  DevCtx* ctx = ctx_for_current();
+  if (!ctx) return LA_ERR_NODEVICE;
+  std::lock_guard<std::mutex> g(mp_mutex(ctx));
+  char* buf = nullptr;
+  int rc = mp_buffer(ctx, al256(sizeof(double) * (size_t)n) + al256(sizeof(unsigned) * tri_chain_ctl_words(n)) + 256,
+                     &buf);
+  if (rc) return rc;
+  double* d_x = reinterpret_cast<double*>(buf);
+  unsigned* ctl = reinterpret_cast<unsigned*>(buf + al256(sizeof(double) * (size_t)n));
+  int* err = reinterpret_cast<int*>(buf + al256(sizeof(double) * (size_t)n) +
+                                    al256(sizeof(unsigned) * tri_chain_ctl_words(n)));
+  const int kase1 = onenrm ? 1 : 2;
+  std::vector<double> v(n), x(n);
+  std::vector<int> isgn(n);
+  int kase = 0, isave[3] = {0, 0, 0};
+  double ainvnm = 0.0;
+  launch_fill_i32(err, 1, 0, s);
+  while (true) {
+    lacn2_host(n, v.data(), x.data(), isgn.data(), &ainvnm, &kase, isave);
+    if (kase == 0) break;
+    MP_CK(hipMemcpyAsync(d_x, x.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (kase == kase1) {
+      launch_tri_chain<double>(0, n, 1, AF, ldaf, d_x, n, ctl, err, s);  // inv(L)
+      launch_tri_chain<double>(1, n, 1, AF, ldaf, d_x, n, ctl, err, s);  // inv(U)
+    } else {
+      launch_tri_chain<double>(2, n, 1, AF, ldaf, d_x, n, ctl, err, s);  // inv(U^T)
+      launch_tri_chain<double>(3, n, 1, AF, ldaf, d_x, n, ctl, err, s);  // inv(L^T)
+    }
+    int e = 0;
+    MP_CK(hipMemcpyAsync(x.data(), d_x, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    MP_CK(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    MP_CK(hipStreamSynchronize(s));
+    if (e) return LA_ERR_SYNC;
+    for (int i = 0; i < n; ++i)
+      if (!std::isfinite(x[i])) return 0;  // DLATRS would scale to zero: RCOND = 0
+  }
+  if (ainvnm != 0.0) {
+    *rcond = (1.0 / ainvnm) / anorm;
+  } else {
+    return 1;
+  }
+  if (std::isnan(*rcond) || *rcond > 1.79769313486231571e308) return 1;
+  return 0;
+}
+
+// ============================================================================
+// DGETRI: inverse from the factors of DGETRF (SRC/dgetri.f:114-263)
+// ============================================================================
+// INFO = i when U(i,i) is exactly zero (dgetri.f:150-155), A untouched then.
+// Otherwise inv(A) = inv(U) inv(L) P is formed as the solution of
+// A X = I with the saved factors -- the blocked triangular solves, whose
+// products run on the FP64 matrix cores -- in a device buffer, then copied
+// over A.  The reference forms inv(U) in place and solves inv(A) L = inv(U);
+// the results agree to rounding (DGET03's ratio, TESTING/LIN/dget03.f).
+int getri_device(int n, double* A, long lda, const int* ipiv, hipStream_t s) {
+  if (n == 0) return 0;
+  DevCtx* ctx = ctx_for_current();
+  if (!ctx) return LA_ERR_NODEVICE;
+  std::lock_guard<std::mutex> g(mp_mutex(ctx));
+  std::vector<double> diag(n);
+  MP_CK(hipMemcpy2DAsync(diag.data(), sizeof(double), A, sizeof(double) * (size_t)(lda + 1), sizeof(double), n,
+                         hipMemcpyDeviceToHost, s));
+  MP_CK(hipStreamSynchronize(s));
+  for (int i = 0; i < n; ++i)
+    if (diag[i] == 0.0) return i + 1;
+  char* buf = nullptr;
+  int rc = mp_buffer(ctx, al256(sizeof(double) * (size_t)n * n), &buf);
+  if (rc) return rc;
+  double* W = reinterpret_cast<double*>(buf);
+  launch_identity(n, W, n, s);
+  rc = with_safe_forms([&]() { return getrs_device<double>('N', n, n, A, lda, ipiv, W, n, s); });
+  if (rc) return rc;
+  launch_copy_d(n, n, W, n, A, lda, s);
+  MP_CK(hipStreamSynchronize(s));
+  return 0;
+}
+
+
+// ============================================================================
+// DLANGE on a device matrix (SRC/dlange.f:143-200)
+// ============================================================================
+// 'M' max |a|, '1'/'O' max column sum, 'I' max row sum (sums in the
+// reference's order, so bitwise its results), 'F'/'E' Frobenius norm as
+// scale * sqrt(sum (a/scale)^2) with scale = max |a| (the reference's DLASSQ
+// accumulates with other scalings: equal to rounding).  The maximum over
+// columns / rows follows the reference's NaN rule (a NaN wins).
+int lange_device(char norm, int m, int n, const double* A, long lda, double* value, hipStream_t s) {
+  *value = 0.0;
+  if (std::min(m, n) == 0) return 0;
+  DevCtx* ctx = ctx_for_current();
+  if (!ctx) return LA_ERR_NODEVICE;
+  std::lock_guard<std::mutex> g(mp_mutex(ctx));
+  const int len = std::max(m, n);
+  char* buf = nullptr;
+  int rc = mp_buffer(ctx, al256(sizeof(double) * (size_t)len), &buf);
+  if (rc) return rc;
+  double* d_v = reinterpret_cast<double*>(buf);
+  std::vector<double> h(len);
+  auto fold_max = [&](int cnt) {
+    double v = 0.0;
+    for (int i = 0; i < cnt; ++i)
+      if (v < h[i] || std::isnan(h[i])) v = h[i];
+    return v;
+  };
+  auto fetch = [&](int cnt) -> int {
+    MP_CK(hipMemcpyAsync(h.data(), d_v, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+    MP_CK(hipStreamSynchronize(s));
+    return 0;
+  };
+  const char c = (char)(norm | 0x20);
+  if (c == 'm' || c == 'f' || c == 'e') {
+    launch_col_norm(m, n, A, lda, 0, 1.0, d_v, s);
+    if ((rc = fetch(n))) return rc;
+    const double amax = fold_max(n);
+    if (c == 'm' || !(amax > 0.0) || !std::isfinite(amax)) {
+      *value = amax;
+      return 0;
+    }
+    launch_col_norm(m, n, A, lda, 2, amax, d_v, s);
+    if ((rc = fetch(n))) return rc;
+    double ssq = 0.0;
+    for (int j = 0; j < n; ++j) ssq += h[j];
+    *value = amax * std::sqrt(ssq);
+  } else if (c == 'o' || c == '1') {
+    launch_col_norm(m, n, A, lda, 1, 1.0, d_v, s);
+    if ((rc = fetch(n))) return rc;
+    *value = fold_max(n);
+  } else if (c == 'i') {
+    launch_row_abssum(m, n, A, lda, d_v, s);
+    if ((rc = fetch(m))) return rc;
+    *value = fold_max(m);
+  }
+  return 0;
 }
 
 }  // namespace la

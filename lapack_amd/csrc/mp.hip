@@ -144,6 +144,48 @@ __global__ __launch_bounds__(256) void col_absmax_scaled_kernel(int m, int n, co
   if (threadIdx.x == 0) c[j] = s[0];
 }
 
+// Per-column quantities of DLANGE (SRC/dlange.f:143-200), one thread per
+// column walking it in row order (the reference's order, so sums are
+// bitwise the reference's): mode 0 max |a| (NaN if the column holds one),
+// 1 sum |a|, 2 sum (a/scale)^2, 3 max |a| over the upper triangle (DLANTR
+// 'M', 'U', 'N', SRC/dlantr.f:170-180).
+__global__ __launch_bounds__(256) void col_norm_kernel(int m, int n, const double* __restrict__ A, long lda,
+                                                       int mode, double scale, double* __restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const double* col = A + (long)j * lda;
+  double v = 0.0;
+  if (mode == 0) {
+    for (int i = 0; i < m; ++i) {
+      const double t = fabs(col[i]);
+      if (v < t || t != t) v = t;
+    }
+  } else if (mode == 3) {  // the upper triangle: rows 0 .. min(m-1, j)
+    const int me = m < j + 1 ? m : j + 1;
+    for (int i = 0; i < me; ++i) {
+      const double t = fabs(col[i]);
+      if (v < t || t != t) v = t;
+    }
+  } else if (mode == 1) {
+    for (int i = 0; i < m; ++i) v = v + fabs(col[i]);
+  } else {
+    for (int i = 0; i < m; ++i) {
+      const double t = col[i] / scale;
+      v = v + t * t;
+    }
+  }
+  out[j] = v;
+}
+
+// A <- I (n x n).
+__global__ __launch_bounds__(256) void identity_kernel(int n, double* __restrict__ A, long lda) {
+  const long tot = (long)n * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
+    const long i = e % n, j = e / n;
+    A[i + j * lda] = i == j ? 1.0 : 0.0;
+  }
+}
+
 // A(i,j) <- c(j)*r(i)*A(i,j) (or one factor): the scaling forms of DLAQGE.
 __global__ __launch_bounds__(256) void scale_rc_kernel(int m, int n, double* __restrict__ A, long lda,
                                                        const double* __restrict__ r, const double* __restrict__ c,
@@ -240,6 +282,12 @@ void launch_col_absmax_scaled(int m, int n, const double* A, long lda, const dou
 }
 void launch_scale_rc(int m, int n, double* A, long lda, const double* r, const double* c, int mode, hipStream_t s) {
   if (m > 0 && n > 0 && mode > 0) scale_rc_kernel<<<grid_for((long)m * n), 256, 0, s>>>(m, n, A, lda, r, c, mode);
+}
+void launch_col_norm(int m, int n, const double* A, long lda, int mode, double scale, double* out, hipStream_t s) {
+  if (n > 0) col_norm_kernel<<<(n + 255) / 256, 256, 0, s>>>(m, n, A, lda, mode, scale, out);
+}
+void launch_identity(int n, double* A, long lda, hipStream_t s) {
+  if (n > 0) identity_kernel<<<grid_for((long)n * n), 256, 0, s>>>(n, A, lda);
 }
 
 }  // namespace la

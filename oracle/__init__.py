@@ -328,6 +328,62 @@ def ref_dget07(a, b, x, xact, ferr, berr, trans="N", chkferr=True):
     return float(out[0]), float(out[1])
 
 
+def ref_dgecon(lu, anorm, norm="1"):
+    """Reference DGECON on the factors lu; returns (RCOND, INFO)."""
+    lu = np.asfortranarray(lu, dtype=np.float64)
+    n = lu.shape[0]
+    work = np.zeros(4 * max(n, 1))
+    iwork = np.zeros(max(n, 1), dtype=np.int32)
+    rcond = ctypes.c_double(0)
+    info = ctypes.c_int(0)
+    r().dgecon_(ctypes.c_char_p(norm.encode()), _ri(n), ctypes.c_void_p(_p(lu)), _ri(max(1, n)),
+                ctypes.byref(ctypes.c_double(anorm)), ctypes.byref(rcond), ctypes.c_void_p(_p(work)),
+                ctypes.c_void_p(_p(iwork)), ctypes.byref(info), ctypes.c_size_t(1))
+    return rcond.value, info.value
+
+
+def ref_dgetri(lu, ipiv):
+    """Reference DGETRI; returns (inverse, INFO)."""
+    a = np.array(lu, dtype=np.float64, order="F", copy=True)
+    n = a.shape[0]
+    ip = np.ascontiguousarray(ipiv, dtype=np.int32)
+    lwork = max(1, 64 * n)
+    work = np.zeros(lwork)
+    info = ctypes.c_int(0)
+    r().dgetri_(_ri(n), ctypes.c_void_p(_p(a)), _ri(max(1, n)), ctypes.c_void_p(_p(ip)), ctypes.c_void_p(_p(work)),
+                _ri(lwork), ctypes.byref(info))
+    return a, info.value
+
+
+def ref_dgesvx(fact, trans, a, b, af=None, ipiv=None, equed="N", rr=None, cc=None):
+    """Reference DGESVX; returns a dict of every output."""
+    a = np.array(a, dtype=np.float64, order="F", copy=True)
+    n = a.shape[0]
+    b = np.array(b, dtype=np.float64, order="F", copy=True).reshape(n, -1, order="F")
+    nrhs = b.shape[1]
+    af = np.zeros((n, n), order="F") if af is None else np.array(af, dtype=np.float64, order="F", copy=True)
+    ip = np.zeros(max(n, 1), dtype=np.int32) if ipiv is None else np.ascontiguousarray(ipiv, dtype=np.int32).copy()
+    rr = np.zeros(max(n, 1)) if rr is None else np.array(rr, dtype=np.float64, copy=True)
+    cc = np.zeros(max(n, 1)) if cc is None else np.array(cc, dtype=np.float64, copy=True)
+    x = np.zeros((n, nrhs), order="F")
+    eq = ctypes.create_string_buffer(equed.encode(), 1)
+    rcond = ctypes.c_double(0)
+    ferr = np.zeros(max(nrhs, 1))
+    berr = np.zeros(max(nrhs, 1))
+    work = np.zeros(4 * max(n, 1))
+    iwork = np.zeros(max(n, 1), dtype=np.int32)
+    info = ctypes.c_int(0)
+    r().dgesvx_(ctypes.c_char_p(fact.encode()), ctypes.c_char_p(trans.encode()), _ri(n), _ri(nrhs),
+                ctypes.c_void_p(_p(a)), _ri(max(1, n)), ctypes.c_void_p(_p(af)), _ri(max(1, n)),
+                ctypes.c_void_p(_p(ip)), eq, ctypes.c_void_p(_p(rr)), ctypes.c_void_p(_p(cc)),
+                ctypes.c_void_p(_p(b)), _ri(max(1, n)), ctypes.c_void_p(_p(x)), _ri(max(1, n)), ctypes.byref(rcond),
+                ctypes.c_void_p(_p(ferr)), ctypes.c_void_p(_p(berr)), ctypes.c_void_p(_p(work)),
+                ctypes.c_void_p(_p(iwork)), ctypes.byref(info), ctypes.c_size_t(1), ctypes.c_size_t(1),
+                ctypes.c_size_t(1))
+    return {"a": a, "af": af, "ipiv": ip[:n], "equed": eq.raw.decode(), "r": rr[:n], "c": cc[:n], "b": b, "x": x,
+            "rcond": rcond.value, "ferr": ferr[:nrhs], "berr": berr[:nrhs], "rpvgrw": work[0], "info": info.value}
+
+
 def ref_sgetrf(a):
     lu = np.array(a, dtype=np.float32, order="F", copy=True)
     m, n = lu.shape

@@ -20,7 +20,8 @@ def declared_functions():
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"\b([a-z_][a-z0-9_]*)\s*\(", text)
     skip = {"if", "sizeof", "defined"}
-    return sorted({n for n in names if n not in skip and (n.endswith("_") or n.startswith("lapack_amd_"))})
+    return sorted({n for n in names if n not in skip and
+                   (n.endswith("_") or n.startswith("lapack_amd_") or n.startswith("LAPACKE_"))})
 
 
 def test_library_exports_every_declared_symbol():
@@ -104,3 +105,62 @@ def test_tuning_hook_roundtrip():
     assert la.get_tuning() == (128, 32)
     la.set_tuning(0, 0)
     assert la.get_tuning() == (256, 64)
+
+
+def test_lapacke_layout_and_nan_checks_cpu():
+    """LAPACKE's checks that come before any device work
+    (LAPACKE/src/lapacke_dgetrf.c:40-51, lapacke_dgesv.c, lapacke_dgetrf_work.c):
+    a bad layout is -1, a NaN in a host input is minus its position, the
+    row-major leading-dimension checks are the _work function's numbers."""
+    lib = la.lib()
+    vp, i = ctypes.c_void_p, ctypes.c_int
+    lib.LAPACKE_dgetrf.restype = i
+    lib.LAPACKE_dgetrf.argtypes = [i, i, i, vp, i, vp]
+    lib.LAPACKE_dgesv.restype = i
+    lib.LAPACKE_dgesv.argtypes = [i, i, i, vp, i, vp, vp, i]
+    lib.LAPACKE_dgetrf_work.restype = i
+    lib.LAPACKE_dgetrf_work.argtypes = [i, i, i, vp, i, vp]
+    lib.LAPACKE_set_nancheck.argtypes = [i]
+    a = np.ones((4, 4))
+    b = np.ones((4, 1))
+    ip = np.zeros(4, dtype=np.int32)
+    assert lib.LAPACKE_dgetrf(7, 4, 4, a.ctypes.data, 4, ip.ctypes.data) == -1
+    a[2, 1] = np.nan
+    assert lib.LAPACKE_dgetrf(102, 4, 4, a.ctypes.data, 4, ip.ctypes.data) == -4
+    assert lib.LAPACKE_dgetrf(101, 4, 4, a.ctypes.data, 4, ip.ctypes.data) == -4
+    a[2, 1] = 1.0
+    b[3, 0] = np.nan
+    assert lib.LAPACKE_dgesv(101, 4, 1, a.ctypes.data, 4, ip.ctypes.data, b.ctypes.data, 1) == -7
+    # row-major: lda below the row length is -5 of the _work function
+    assert lib.LAPACKE_dgetrf_work(101, 4, 4, a.ctypes.data, 3, ip.ctypes.data) == -5
+    # a NaN scan switched off reaches the routine's own checks: LDA -4 shifted to -5
+    # (the default XERBLA stops the program, as SRC/xerbla.f does: record instead)
+    lib.LAPACKE_set_nancheck(0)
+    la.xerbla_record(True)
+    try:
+        assert lib.LAPACKE_dgesv(102, 4, 1, a.ctypes.data, 3, ip.ctypes.data, b.ctypes.data, 4) == -5
+        assert la.xerbla_last()[:2] == ("DGESV", 4)
+    finally:
+        lib.LAPACKE_set_nancheck(1)
+        la.xerbla_record(False)
+
+
+def test_tuning_table_json(tmp_path):
+    """A table written in the tuner's format is read back per bucket (the next
+    power of two of min(m, n)); explicit tuning overrides it."""
+    path = tmp_path / "tune.json"
+    path.write_text('{"getrf": {"16384": {"nb": 192, "leaf": 32, "ms": 101.5}, '
+                    '"1024": {"nb": 128, "leaf": 64, "ms": 1.2}}}\n')
+    assert la.load_tuning(str(path)) == 2
+    assert la.get_tuning_for(16384) == (192, 32)
+    assert la.get_tuning_for(9000) == (192, 32)
+    assert la.get_tuning_for(1000) == (128, 64)
+    assert la.get_tuning_for(3000) == (256, 64)   # no entry for 4096: defaults
+    la.set_tuning(64, 16)
+    try:
+        assert la.get_tuning_for(16384) == (64, 16)
+    finally:
+        la.set_tuning(0, 0)
+    assert la.load_tuning(str(tmp_path / "missing.json")) == -1
+    assert la.load_tuning(None) == 0
+    assert la.get_tuning_for(16384) == (256, 64)
