@@ -237,6 +237,21 @@ int lapack_amd_sgetrs_batched_async(const lapack_amd_batch_plan* plan, const int
                                     const int* d_lda, int* const* d_ipiv, float* const* d_b, const int* d_ldb,
                                     void* stream);
 
+/* Batched adjoint of the solve (the reverse pass of X_b = A_b^-1 B_b for every
+ * system of a 'd' plan), from the saved factors of
+ * lapack_amd_dgetrf_batched_async: Xbar_b is overwritten by
+ * Bbar_b = A_b^-T Xbar_b -- DGETRS('T') semantics on the saved LU and IPIV,
+ * SRC/dgetrs.f:199-217, no refactorization -- and Abar_b <- -Bbar_b X_b^T
+ * (+ Abar_b when accumulate != 0), the rank-NRHS DGEMM('N','T') of
+ * BLAS/SRC/dgemm.f.  The reference has no batched routine.  Two launches
+ * (a persistent transposed solve, one workgroup per system, and the
+ * rank-NRHS product over every system), no allocation, capturable in a
+ * hipGraph.  Returns 0, -1 for a plan that is not FP64, or LA_ERR_LIMIT. */
+int lapack_amd_dgesv_adjoint_batched_async(const lapack_amd_batch_plan* plan, const int* d_nrhs, double* const* d_lu,
+                                           const int* d_lda, int* const* d_ipiv, double* const* d_x, const int* d_ldx,
+                                           double* const* d_xbar, const int* d_ldxbar, double* const* d_abar,
+                                           const int* d_ldabar, int accumulate, void* stream);
+
 /* ====================== 5. LAPACKE C interface ============================ */
 
 /* The LAPACKE functions of the path (LAPACKE/include/lapacke.h; lapack_int

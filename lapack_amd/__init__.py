@@ -82,6 +82,7 @@ def _declare(L):
         "lapack_amd_sgetrf_batched_async": (i, [vp, vp, vp, vp, vp, vp]),
         "lapack_amd_dgetrs_batched_async": (i, [vp, vp, vp, vp, vp, vp, vp, vp]),
         "lapack_amd_sgetrs_batched_async": (i, [vp, vp, vp, vp, vp, vp, vp, vp]),
+        "lapack_amd_dgesv_adjoint_batched_async": (i, [vp] * 11 + [i, vp]),
         "lapack_amd_set_tuning": (None, [i, i]),
         "lapack_amd_get_tuning": (None, [_c_int_p, _c_int_p]),
         "lapack_amd_xerbla_record": (None, [i]),
@@ -315,6 +316,14 @@ class BatchPlan:
         fn = lib().lapack_amd_dgetrs_batched_async if self.precision == "d" else lib().lapack_amd_sgetrs_batched_async
         return fn(self._h, _ptr(d_nrhs), _ptr(d_aptr), _ptr(d_lda), _ptr(d_ipivptr), _ptr(d_bptr), _ptr(d_ldb),
                   _stream(stream))
+
+    def adjoint(self, d_nrhs, d_luptr, d_lda, d_ipivptr, d_xptr, d_ldx, d_xbarptr, d_ldxbar, d_abarptr, d_ldabar,
+                accumulate=False, stream=None):
+        """Batched adjoint: Xbar_b <- A_b^-T Xbar_b, Abar_b <- -Xbar_b X_b^T (+ Abar_b)."""
+        return lib().lapack_amd_dgesv_adjoint_batched_async(
+            self._h, _ptr(d_nrhs), _ptr(d_luptr), _ptr(d_lda), _ptr(d_ipivptr), _ptr(d_xptr), _ptr(d_ldx),
+            _ptr(d_xbarptr), _ptr(d_ldxbar), _ptr(d_abarptr), _ptr(d_ldabar), 1 if accumulate else 0,
+            _stream(stream))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -47,10 +47,11 @@ struct lapack_amd_batch_plan {
   int cfg = -1;     // kernel configuration index
   int grid_f = 1;   // persistent grid of the getrf launch
   int grid_s = 1;   // persistent grid of the getrs launch
+  int grid_t = 1;   // persistent grid of the adjoint's transposed solve (FP64)
   int dev = -1;
   int* d_n = nullptr;
   int* d_order = nullptr;
-  unsigned* d_counter = nullptr;  // [0] getrf, [1] getrs
+  unsigned* d_counter = nullptr;  // [0] getrf, [1] getrs, [2] adjoint
   std::vector<int> n;
   double tuned_ms = 0.0;
 };
@@ -166,6 +167,9 @@ static int rc_grid(lapack_amd_batch_plan* p) {
   if (gs < 0) return gs;
   p->grid_f = gf;
   p->grid_s = gs;
+  // the adjoint's grid; a plan whose orders exceed its LDS keeps 0 (the
+  // adjoint then reports LA_ERR_LIMIT)
+  p->grid_t = p->elem == 8 ? std::max(0, batched_getrs_t_grid<double>(b, nm)) : 0;
   return 0;
 }
 
@@ -312,6 +316,22 @@ int lapack_amd_sgetrs_batched_async(const lapack_amd_batch_plan* p, const int* d
   if (p->batch == 0) return 0;
   int rc = launch_getrs_batched<float>(p->grid_s, p->batch, p->nmax, p->d_n, d_nrhs, d_a, d_lda, d_ipiv, d_b, d_ldb,
                                        p->d_order, p->d_counter + 1, (hipStream_t)stream);
+  return rc ? rc : hipc(hipGetLastError());
+}
+
+
+/* Batched adjoint of the solve (see the header). */
+int lapack_amd_dgesv_adjoint_batched_async(const lapack_amd_batch_plan* p, const int* d_nrhs, double* const* d_lu,
+                                           const int* d_lda, int* const* d_ipiv, double* const* d_x, const int* d_ldx,
+                                           double* const* d_xbar, const int* d_ldxbar, double* const* d_abar,
+                                           const int* d_ldabar, int accumulate, void* stream) {
+  if (!p) return -1;
+  if (p->elem != 8) return -1;
+  if (p->batch == 0) return 0;
+  if (p->grid_t <= 0) return LA_ERR_LIMIT;
+  int rc = launch_adjoint_batched(p->grid_t, p->batch, p->nmax, p->d_n, d_nrhs, d_lu, d_lda, d_ipiv, d_x, d_ldx,
+                                  d_xbar, d_ldxbar, d_abar, d_ldabar, accumulate, p->d_order, p->d_counter + 2,
+                                  (hipStream_t)stream);
   return rc ? rc : hipc(hipGetLastError());
 }
 

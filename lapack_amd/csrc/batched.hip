@@ -405,6 +405,204 @@ __global__ __launch_bounds__(NT) void getrs_batched_kernel(int batch, const int*
   }
 }
 
+// DGETRS('T') for one matrix per workgroup, NR right-hand sides at a time
+// (SRC/dgetrs.f:199-217): U^T Y = B (forward), L^T X = Y (backward, unit
+// diagonal), then the interchanges in reverse order.  The right-hand sides
+// live in LDS.  Each 64-row block first takes the contributions of the rows
+// already solved as dot products down columns of A (row i of U^T is column
+// i of A, read coalesced, one wavefront per column, a fixed-order wavefront
+// reduction), then one wavefront solves the block's 64 x 64 triangle from a
+// copy staged in LDS.
+template <class T, int NT, int NR>
+__global__ __launch_bounds__(NT) void getrs_t_batched_kernel(int batch, const int* __restrict__ nv,
+                                                             const int* __restrict__ nrhsv, T* const* __restrict__ av,
+                                                             const int* __restrict__ ldav, int* const* __restrict__ ipv,
+                                                             T* const* __restrict__ bv, const int* __restrict__ ldbv,
+                                                             const int* __restrict__ order, unsigned* counter,
+                                                             int ldx) {
+  constexpr int NW = NT / 64;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* X = reinterpret_cast<T*>(smem_raw);  // X[r + q*ldx]
+  __shared__ T sD[64][65];                // sD[c][r] = A(kb + r, kb + c)
+  __shared__ int s_item;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int guard = 0; guard <= batch; ++guard) {
+    if (tid == 0) s_item = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int item = s_item;
+    if (item >= batch) break;
+    const int b = order[item];
+    const int n = nv[b], nrhs = nrhsv[b];
+    const T* A = av[b];
+    const long lda = ldav[b];
+    const int* ipiv = ipv[b];
+    T* B = bv[b];
+    const long ldb = ldbv[b];
+    auto stage_diag = [&](int kb, int bs) {
+      for (int e = tid; e < 64 * 64; e += NT) {
+        const int r = e & 63, c = e >> 6;
+        sD[c][r] = (r < bs && c < bs) ? A[(kb + r) + (long)(kb + c) * lda] : T(0);
+      }
+    };
+    for (int q0 = 0; q0 < nrhs; q0 += NR) {
+      const int nr = min(NR, nrhs - q0);
+      for (int q = 0; q < nr; ++q)
+        for (int r = tid; r < n; r += NT) X[r + q * ldx] = B[r + (long)(q0 + q) * ldb];
+      __syncthreads();
+      // U^T Y = B (dgetrs.f:205)
+      for (int kb = 0; kb < n; kb += 64) {
+        const int bs = min(64, n - kb);
+        for (int c = wave; c < bs; c += NW) {
+          const int i = kb + c;
+          T acc[NR];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) acc[q] = T(0);
+          for (int k = lane; k < kb; k += 64) {
+            const T u = A[k + (long)i * lda];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[q] = acc[q] + u * ((q < nr) ? X[k + q * ldx] : T(0));
+          }
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc[q] = acc[q] + __shfl_xor(acc[q], off);
+          if (lane == 0)
+            for (int q = 0; q < nr; ++q) X[i + q * ldx] = X[i + q * ldx] - acc[q];
+        }
+        stage_diag(kb, bs);
+        __syncthreads();
+        if (wave == 0) {
+          T xi[NR];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) xi[q] = (q < nr && lane < bs) ? X[kb + lane + q * ldx] : T(0);
+          for (int k = 0; k < bs; ++k) {
+            const T dkk = sD[k][k];
+            const T u = (lane > k && lane < bs) ? sD[lane][k] : T(0);  // U(kb+k, kb+lane)
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+              if (lane == k) xi[q] = xi[q] / dkk;
+              const T xk = __shfl(xi[q], k);
+              if (lane > k && lane < bs) xi[q] = xi[q] - u * xk;
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+            if (q < nr && lane < bs) X[kb + lane + q * ldx] = xi[q];
+        }
+        __syncthreads();
+      }
+      // L^T X = Y, unit diagonal (dgetrs.f:211)
+      for (int kb = ((n - 1) / 64) * 64; kb >= 0; kb -= 64) {
+        const int bs = min(64, n - kb);
+        for (int c = wave; c < bs; c += NW) {
+          const int i = kb + c;
+          T acc[NR];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) acc[q] = T(0);
+          for (int k = kb + bs + lane; k < n; k += 64) {
+            const T l = A[k + (long)i * lda];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[q] = acc[q] + l * ((q < nr) ? X[k + q * ldx] : T(0));
+          }
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc[q] = acc[q] + __shfl_xor(acc[q], off);
+          if (lane == 0)
+            for (int q = 0; q < nr; ++q) X[i + q * ldx] = X[i + q * ldx] - acc[q];
+        }
+        stage_diag(kb, bs);
+        __syncthreads();
+        if (wave == 0) {
+          T xi[NR];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) xi[q] = (q < nr && lane < bs) ? X[kb + lane + q * ldx] : T(0);
+          for (int k = bs - 1; k >= 0; --k) {
+            const T l = (lane < k) ? sD[lane][k] : T(0);  // L(kb+k, kb+lane)
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+              const T xk = __shfl(xi[q], k);
+              if (lane < k) xi[q] = xi[q] - l * xk;
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+            if (q < nr && lane < bs) X[kb + lane + q * ldx] = xi[q];
+        }
+        __syncthreads();
+      }
+      if (tid < nr) {  // interchanges in reverse order (dgetrs.f:217)
+        for (int k = n - 1; k >= 0; --k) {
+          int p = ipiv[k] - 1;
+          if (p < 0 || p >= n) p = k;
+          if (p != k) { const T t = X[k + tid * ldx]; X[k + tid * ldx] = X[p + tid * ldx]; X[p + tid * ldx] = t; }
+        }
+      }
+      __syncthreads();
+      for (int q = 0; q < nr; ++q)
+        for (int r = tid; r < n; r += NT) B[r + (long)(q0 + q) * ldb] = X[r + q * ldx];
+      __syncthreads();
+    }
+  }
+}
+
+// Abar_b <- -Bbar_b X_b^T (+ Abar_b when accumulate), for every matrix b of
+// the batch: the adjoint's rank-nrhs product, bound by the n^2 result
+// written to HBM.  Grid (tiles of the largest order, batch): a 64 x 64 tile
+// per 256-thread workgroup, 32 of the nrhs columns at a time staged in LDS,
+// each lane writing 16 columns of one row; tiles past a matrix's order exit.
+__global__ __launch_bounds__(256) void rankk_batched_kernel(const int* __restrict__ nv, const int* __restrict__ nrhsv,
+                                                            double* const* __restrict__ bbv,
+                                                            const int* __restrict__ ldbbv,
+                                                            double* const* __restrict__ xv, const int* __restrict__ ldxv,
+                                                            double* const* __restrict__ cv,
+                                                            const int* __restrict__ ldcv, int tiles1, int accumulate) {
+  constexpr int TL = 64, KC = 32;
+  __shared__ double sa[KC][TL];
+  __shared__ double sb[KC][TL + 1];
+  const int b = blockIdx.y;
+  const int n = nv[b];
+  const int i0 = (blockIdx.x % tiles1) * TL, j0 = (blockIdx.x / tiles1) * TL;
+  if (i0 >= n || j0 >= n) return;
+  const int k = nrhsv[b];
+  const double* A = bbv[b];
+  const long lda = ldbbv[b];
+  const double* Bm = xv[b];
+  const long ldbm = ldxv[b];
+  double* C = cv[b];
+  const long ldc = ldcv[b];
+  const int t = threadIdx.x;
+  const int r = t % TL, cg = t / TL;
+  double acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+  for (int p0 = 0; p0 < k; p0 += KC) {
+    const int kc = min(KC, k - p0);
+    __syncthreads();
+    for (int e = t; e < kc * TL; e += 256) {
+      const int p = e / TL, rr = e % TL;
+      sa[p][rr] = (i0 + rr < n) ? A[(i0 + rr) + (long)(p0 + p) * lda] : 0.0;
+      sb[p][rr] = (j0 + rr < n) ? Bm[(j0 + rr) + (long)(p0 + p) * ldbm] : 0.0;
+    }
+    __syncthreads();
+    for (int p = 0; p < kc; ++p) {
+      const double a = sa[p][r];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q] = fma(a, sb[p][cg * 16 + q], acc[q]);
+    }
+  }
+  const int i = i0 + r;
+  if (i >= n) return;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int j = j0 + cg * 16 + q;
+    if (j < n) {
+      double* cp = C + i + (long)j * ldc;
+      *cp = accumulate ? *cp - acc[q] : -acc[q];
+    }
+  }
+}
+
 // uniform(-1,1) from a hash of the element index: synthetic input of the
 // autotuner's timing runs.
 template <class T>
@@ -540,6 +738,46 @@ int launch_getrs_batched(int grid, int batch, int nmax, const int* nv, const int
                                                       std::max(nmax, 1));
   return 0;
 }
+template <class T>
+int batched_getrs_t_grid(int batch, int nmax) {
+  constexpr int NT = 256, NR = 4;
+  const size_t sm = sizeof(T) * (size_t)NR * std::max(nmax, 1);
+  if (sm + sizeof(T) * 64 * 65 + 64 > (size_t)lds_limit()) return LA_ERR_LIMIT;
+  auto kern = getrs_t_batched_kernel<T, NT, NR>;
+  if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit()) != hipSuccess)
+    (void)hipGetLastError();
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NT, sm) != hipSuccess || occ < 1) {
+    (void)hipGetLastError();
+    occ = 1;
+  }
+  int dev = 0, ncu = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  return std::max(1, std::min(batch, ncu * occ));
+}
+template int batched_getrs_t_grid<double>(int, int);
+
+// The batched adjoint: Bbar = A^-T Xbar in place of Xbar (getrs_t_batched),
+// then Abar = -Bbar X^T (+ Abar).
+int launch_adjoint_batched(int grid, int batch, int nmax, const int* nv, const int* nrhsv, double* const* luv,
+                           const int* ldluv, int* const* ipv, double* const* xv, const int* ldxv,
+                           double* const* xbv, const int* ldxbv, double* const* abv, const int* ldabv,
+                           int accumulate, const int* order, unsigned* counter, hipStream_t s) {
+  constexpr int NT = 256, NR = 4;
+  const size_t sm = sizeof(double) * (size_t)NR * std::max(nmax, 1);
+  launch_fill_u32(counter, 1, 0u, s);
+  getrs_t_batched_kernel<double, NT, NR><<<grid, NT, sm, s>>>(batch, nv, nrhsv, luv, ldluv, ipv, xbv, ldxbv, order,
+                                                             counter, std::max(nmax, 1));
+  const int tiles1 = (nmax + 63) / 64;
+  for (int b0 = 0; b0 < batch; b0 += 65535) {
+    const int nb = std::min(65535, batch - b0);
+    rankk_batched_kernel<<<dim3(tiles1 * tiles1, nb), 256, 0, s>>>(nv + b0, nrhsv + b0, xbv + b0, ldxbv + b0, xv + b0,
+                                                                 ldxv + b0, abv + b0, ldabv + b0, tiles1, accumulate);
+  }
+  return 0;
+}
+
 template int launch_getrs_batched<double>(int, int, int, const int*, const int*, double* const*, const int*,
                                           int* const*, double* const*, const int*, const int*, unsigned*, hipStream_t);
 template int launch_getrs_batched<float>(int, int, int, const int*, const int*, float* const*, const int*,

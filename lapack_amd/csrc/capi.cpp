@@ -848,6 +848,13 @@ int gemm_sync(char ta, char tb, int m, int n, int k, double alpha, const double*
 
 }  // namespace
 
+namespace la {
+// DGETRS's argument checks (SRC/dgetrs.f:157-174) for other sources.
+int getrs_arg_check(char trans, int n, int nrhs, int lda, int ldb) {
+  return getrs_check<double>(trans, n, nrhs, lda, ldb);
+}
+}  // namespace la
+
 extern "C" {
 
 // ---- XERBLA -----------------------------------------------------------------

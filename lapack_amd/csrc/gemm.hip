@@ -366,6 +366,209 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
   }
 }
 
+// Buffer-resource access (MUBUF): a uniform 64-bit base in scalar registers
+// plus a 32-bit per-lane byte offset and a uniform byte offset, so a tile's
+// loads and stores keep one offset register per lane instead of one 64-bit
+// address per element.  The range is never checked (num_records = 2^31-1;
+// every offset used stays inside the tile's own matrix).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, 0x7fffffff, 0x00020000);
+}
+typedef int la_i2 __attribute__((ext_vector_type(2)));
+template <class T> __device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <> __device__ __forceinline__ double buf_ld<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <> __device__ __forceinline__ float buf_ld<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_st(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(la_i2, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, voff, soff, 0);
+}
+
+// The trailing update C <- C - A*B, persistent and software-pipelined
+// across tiles, for M and N multiples of 128 (the launcher gives any edge
+// strips to gemm_kernel).  A 512-thread workgroup (8 waves, 4 x 2) owns a
+// 128 x 128 tile at a time, each wave 32 x 64 = 2 x 4 MFMA blocks (32 FP64
+// accumulators per lane), which leaves registers for a second set: while a
+// tile's k loop runs, the C elements of the workgroup's next tile are read
+// into that set, one block per k step, issued after the step's A/B loads so
+// that waiting for those never waits for C.  In the tile's last k step the
+// A/B loads fetch the next tile's first k slice.  A tile's end is then its C
+// stores and a register copy; no read of C or of the first slice is waited
+// for between tiles.  BK = 32 per LDS stage (137 KB of LDS for FP64, one
+// workgroup per CU).  Every global access is a buffer access from a uniform
+// base (the tile's corner, or the k slice's) plus a per-lane and a uniform
+// byte offset, whose sum stays below 2^31 while ld * sizeof(T) < 2^24 (the
+// launcher checks this).
+template <class T>
+__global__ __launch_bounds__(512) void update_pipe_kernel(int K, const T* __restrict__ A, long lda,
+                                                         const T* __restrict__ B, long ldb, T* __restrict__ C,
+                                                         long ldc, int tm, int tn, int chunk, int nxcd) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int NTB = 512, TBM = 128, TBN = 128, PBK = 32;
+  constexpr int AST = TBM + Mfma<T>::APAD;
+  constexpr int BST = PBK + 1;
+  constexpr int LA = TBM * PBK / NTB, LB = TBN * PBK / NTB;
+  constexpr int NG = 8;  // accumulator blocks per wave = C read groups per tile
+  constexpr int ES = sizeof(T);
+  __shared__ T As[2][PBK * AST];
+  __shared__ T Bs[2][TBN * BST];
+
+  const int bid = blockIdx.x;
+  int xcd = bid % NXCD;
+  if (nxcd < NXCD) {
+    if (xcd < NXCD - nxcd) return;
+    xcd -= NXCD - nxcd;
+  }
+  const int per = gridDim.x / NXCD;
+  const int ntiles = tm * tn;
+  int slot = bid / NXCD;
+  if (slot >= chunk || xcd * chunk + slot >= ntiles) return;
+  auto tile_origin = [&](int s, int& r0, int& c0) {
+    const int lin = xcd * chunk + s;
+    const int width = GROUP_M * tn;
+    const int group = lin / width;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(tm - first_m, GROUP_M);
+    r0 = (first_m + (lin % width) % gsize) * TBM;
+    c0 = ((lin % width) / gsize) * TBN;
+  };
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave & 3;   // 32-row band of the tile
+  const int wn = wave >> 2;  // 64-column band
+  const int li = lane & 15, lk = lane >> 4;
+
+  acc_t acc[2][4], cb[2][4];
+  T ra[LA], rb[LB];
+  const int nkt = (K + PBK - 1) / PBK;
+  // A slice: thread (ai, ak) loads row ai, k = ak + 4q (ak uniform per wave)
+  const int ai = tid % TBM;
+  const int ak = wave / 2;
+  const int va = ai * ES;
+  // B slice: thread (bk, bj) loads k = bk of columns bj + 16q
+  const int bk = tid % PBK, bj = tid / PBK;
+  const int vb = (int)((bk + bj * ldb) * ES);
+  auto load_tile = [&](int r0, int c0, int kt) {
+    const int k0 = kt * PBK;
+    const __amdgpu_buffer_rsrc_t ra_ = buf_rsrc(A + (long)r0 + (long)k0 * lda);
+    const __amdgpu_buffer_rsrc_t rb_ = buf_rsrc(B + (long)k0 + (long)c0 * ldb);
+    if (k0 + PBK <= K) {
+#pragma unroll
+      for (int q = 0; q < LA; ++q) ra[q] = buf_ld<T>(ra_, va, (int)((ak + 4 * q) * lda * ES));
+#pragma unroll
+      for (int q = 0; q < LB; ++q) rb[q] = buf_ld<T>(rb_, vb, (int)(16 * q * ldb * ES));
+    } else {
+      // the last, partial k slice: k past K reads as zero
+#pragma unroll
+      for (int q = 0; q < LA; ++q)
+        ra[q] = (k0 + ak + 4 * q < K) ? buf_ld<T>(ra_, va, (int)((ak + 4 * q) * lda * ES)) : T(0);
+      const bool kin = k0 + bk < K;
+      const int vbk = (int)((min(bk, K - 1 - k0) + bj * ldb) * ES);
+#pragma unroll
+      for (int q = 0; q < LB; ++q) {
+        const T v = buf_ld<T>(rb_, vbk, (int)(16 * q * ldb * ES));
+        rb[q] = kin ? v : T(0);
+      }
+    }
+  };
+  auto store_tile = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < LA; ++q) As[buf][(ak + 4 * q) * AST + ai] = -ra[q];
+#pragma unroll
+    for (int q = 0; q < LB; ++q) Bs[buf][(bj + 16 * q) * BST + bk] = rb[q];
+  };
+  // C elements of block (i, j) for this lane: register r holds row
+  // r0 + wm*32 + 16i + li, column c0 + wn*64 + 16j + row(lk, r): one
+  // per-lane offset, the rest uniform
+  const int vc = (int)((wm * 32 + li + (long)(wn * 64 + Mfma<T>::row(lk, 0)) * ldc) * ES);
+  auto c_soff = [&](int i, int j, int r) { return (int)((i * 16 + (long)(j * 16 + Mfma<T>::row(0, r)) * ldc) * ES); };
+  auto read_c = [&](acc_t& dst, int r0, int c0, int i, int j) {
+    const __amdgpu_buffer_rsrc_t rc = buf_rsrc(C + (long)r0 + (long)c0 * ldc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dst[r] = buf_ld<T>(rc, vc, c_soff(i, j, r));
+  };
+
+  int row0, col0;
+  tile_origin(slot, row0, col0);
+  load_tile(row0, col0, 0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) read_c(acc[i][j], row0, col0, i, j);
+  store_tile(0);
+  __syncthreads();
+  int it = 0;
+#pragma nounroll
+  for (;;) {
+    const int nslot = slot + per;
+    const bool more = nslot < chunk && xcd * chunk + nslot < ntiles;
+    int row1 = 0, col1 = 0;
+    if (more) tile_origin(nslot, row1, col1);
+#pragma nounroll
+    for (int kt = 0; kt < nkt; ++kt, ++it) {
+      const int cur = it & 1;
+      const bool nxt = kt + 1 < nkt || more;
+      if (kt + 1 < nkt) load_tile(row0, col0, kt + 1);
+      else if (more) load_tile(row1, col1, 0);
+      if (more) {
+        // the next tile's C, group g read at step g * nkt / NG
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+          if ((g * nkt) / NG == kt) read_c(cb[g / 4][g % 4], row1, col1, g / 4, g % 4);
+      }
+      const T* as = As[cur];
+      const T* bs = Bs[cur];
+      // operand fragments one 4-deep k step ahead of the MFMAs that use
+      // them; the scheduling barrier keeps the compiler from hoisting every
+      // step's LDS reads to the top (which costs 12 registers per step)
+      T af[2][2], bf[2][4];
+      auto frag = [&](int kk, int u) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b) af[u][b] = as[(kk * 4 + lk) * AST + wm * 32 + b * 16 + li];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) bf[u][b] = bs[(wn * 64 + b * 16 + li) * BST + kk * 4 + lk];
+      };
+      frag(0, 0);
+#pragma unroll
+      for (int kk = 0; kk < PBK / 4; ++kk) {
+        if (kk + 1 < PBK / 4) frag(kk + 1, (kk + 1) & 1);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(bf[kk & 1][j], af[kk & 1][i], acc[i][j]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      if (nxt) store_tile(cur ^ 1);
+      __syncthreads();
+    }
+    // this tile's results (C was read into the accumulators)
+    {
+      const __amdgpu_buffer_rsrc_t rc = buf_rsrc(C + (long)row0 + (long)col0 * ldc);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) buf_st(acc[i][j][r], rc, vc, c_soff(i, j, r));
+    }
+    if (!more) break;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = cb[i][j];
+    slot = nslot;
+    row0 = row1;
+    col0 = col1;
+  }
+}
+
 // Small products (C = alpha*A*B + beta*C, A and B untransposed, few 128x128
 // tiles: the DTRSM product of the next panel's 256 columns is 256x256x256,
 // four such tiles): 64x64 tiles, so that 16 workgroups share the work.  A
@@ -553,6 +756,17 @@ static int update_small_tiles() {
   return v;
 }
 
+// Large updates on the software-pipelined kernel (LAPACK_AMD_UPD_PIPE=1) or
+// the 256x128 one-tile-per-workgroup kernel (0).
+static int update_pipe() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("LAPACK_AMD_UPD_PIPE");
+    v = (e && *e) ? std::atoi(e) : 0;
+  }
+  return v;
+}
+
 template <class T>
 void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long ldb, T* C, long ldc, int nskip,
                      hipStream_t s) {
@@ -564,6 +778,30 @@ void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long
   // XCD, so the panel's own short launches find CUs at once instead of
   // queueing behind the update's workgroups.
   const int tmb = (m + 255) / 256, tnb = (n + 127) / 128;
+  const long ldmax = std::max(lda, std::max(ldb, ldc)) * (long)sizeof(T);
+  if (update_pipe() && (long)tmb * tnb >= 512 && ldmax < (1L << 24)) {
+    // the 128 x 128 tiles inside C on the pipelined kernel, the edge strips
+    // (fewer than 128 rows or columns) on gemm_kernel
+    const int tmp = m / 128, tnp = n / 128;
+    const int mi = tmp * 128, ni = tnp * 128;
+    const int chunk = (tmp * tnp + nx - 1) / nx;
+    int per = std::min(chunk, 32);
+    if (nskip > 0 && 2.0 * m * (double)n * k < update_persist_flops()) per = std::min(per, sizeof(T) == 4 ? update_cus_per_xcd_s() : update_cus_per_xcd());
+    update_pipe_kernel<T><<<dim3(per * NXCD), 512, 0, s>>>(k, A, lda, B, ldb, C, ldc, tmp, tnp, chunk, nx);
+    if (ni < n) {
+      const int tm1 = (m + BM - 1) / BM, tn1 = (n - ni + BN - 1) / BN;
+      const int ch1 = (tm1 * tn1 + nx - 1) / nx;
+      gemm_kernel<T, false, false, true><<<dim3(ch1 * NXCD), NT, 0, s>>>(m, n - ni, k, T(-1), A, lda, B + (long)ni * ldb,
+                                                                       ldb, T(1), C + (long)ni * ldc, ldc, tm1, tn1, ch1, nx);
+    }
+    if (mi < m) {
+      const int tm1 = (m - mi + BM - 1) / BM, tn1 = (ni + BN - 1) / BN;
+      const int ch1 = (tm1 * tn1 + nx - 1) / nx;
+      gemm_kernel<T, false, false, true><<<dim3(ch1 * NXCD), NT, 0, s>>>(m - mi, ni, k, T(-1), A + mi, lda, B, ldb, T(1),
+                                                                       C + mi, ldc, tm1, tn1, ch1, nx);
+    }
+    return;
+  }
   if ((long)tmb * tnb >= 512) {
     const int chunk = (tmb * tnb + nx - 1) / nx;
     int per = chunk;

@@ -191,12 +191,25 @@ template <class T>
 int launch_getrs_batched(int grid, int batch, int nmax, const int* nv, const int* nrhsv, T* const* av,
                          const int* ldav, int* const* ipv, T* const* bv, const int* ldbv, const int* order,
                          unsigned* counter, hipStream_t s);
+template <class T> int batched_getrs_t_grid(int batch, int nmax);
+int launch_adjoint_batched(int grid, int batch, int nmax, const int* nv, const int* nrhsv, double* const* luv,
+                           const int* ldluv, int* const* ipv, double* const* xv, const int* ldxv,
+                           double* const* xbv, const int* ldxbv, double* const* abv, const int* ldabv,
+                           int accumulate, const int* order, unsigned* counter, hipStream_t s);
 
 // Triangular sweeps of DGETRS in one launch each (getrs.hip).  which: 0 = L
 // (unit lower), 1 = U, 2 = U^T, 3 = L^T; X is n x nrhs (ldx), solved in place.
 // ctl: tri_chain_ctl_words(n) unsigned of scratch (cleared by the launcher);
 // err: set to 1 if a wait timed out.
 size_t tri_chain_ctl_words(int n);
+// The same sweep as a chain workgroup plus a helper per 64-row block
+// (getrs.hip): ctl tri_sweep_ctl_words(n) unsigned, G tri_sweep_g_elems(n)
+// elements of scratch; err as above.
+size_t tri_sweep_ctl_words(int n);
+size_t tri_sweep_g_elems(int n);
+template <class T>
+void launch_tri_sweep(int which, int n, int nrhs, const T* A, long lda, T* X, long ldx, unsigned* ctl, T* G,
+                      int* err, hipStream_t s);
 template <class T>
 void launch_tri_chain(int which, int n, int nrhs, const T* A, long lda, T* X, long ldx, unsigned* ctl, int* err,
                       hipStream_t s);

@@ -711,6 +711,30 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
 
 // --------------------------------------------------------------- GETRS --
 
+// DGETRS's triangular sweeps on tri_sweep (LAPACK_AMD_GETRS_SWEEP=1) or on
+// tri_chain (0, the default).
+static bool getrs_sweep() {
+  static int v = -1;
+  if (v < 0) v = env_int("LAPACK_AMD_GETRS_SWEEP", 0);
+  return v != 0;
+}
+
+// Workspace of the chain form of DGETRS (getrs_chain); sweep: the
+// chain-and-helpers sweep kernel (its G tiles: nblk * 3 * 64 * 64 elements)
+// rather than one workgroup per block.
+template <class T>
+size_t getrs_chain_ws_bytes(int n, int nrhs, bool sweep) {
+  Carve cv;
+  cv.take<int>((size_t)n);
+  cv.take<T>((size_t)n * nrhs);
+  cv.take<unsigned>(sweep ? tri_sweep_ctl_words(n) : tri_chain_ctl_words(n));
+  cv.take<int>(1);
+  cv.take<int>(perm_pairs_ints(0, n));
+  if (sweep) cv.take<T>(tri_sweep_g_elems(n));
+  return cv.off;
+}
+
+// Workspace of the blocked form (a redone call).
 template <class T>
 size_t getrs_ws_bytes(int n, int nrhs, int nbs) {
   Carve cv;
@@ -719,19 +743,60 @@ size_t getrs_ws_bytes(int n, int nrhs, int nbs) {
   cv.take<T>((size_t)nbs * nrhs);
   cv.take<T>((size_t)nbs * (size_t)((n + nbs - 1) / nbs) * nbs);
   cv.take<T>((size_t)nbs * (size_t)((n + nbs - 1) / nbs) * nbs);
-  cv.take<unsigned>(tri_chain_ctl_words(n));
-  cv.take<int>(1);
   cv.take<int>(perm_pairs_ints(0, n));
-  return cv.off;
+  return std::max(cv.off, getrs_chain_ws_bytes<T>(n, nrhs, true));
+}
+
+// The chain form of DGETRS (SRC/dgetrs.f:181-218) on the device workspace
+// ws (getrs_chain_ws_bytes(n, nrhs, sweep) bytes, 256-byte aligned): the row
+// permutation, then one launch per triangular sweep -- tri_sweep (a chain
+// workgroup and a helper per 64-row block) when sweep, else tri_chain (a
+// workgroup per block in ticket order; getrs.hip).  When a wait gives up, B
+// is left as passed and the failure is folded into *waitflag and *sticky
+// (device ints; either may be null).  fail: start with the wait-error word
+// set (the test hook).  Enqueued on s; allocates nothing.
+template <class T>
+void getrs_chain(char trans, int n, int nrhs, const T* A, long lda, const int* ipiv, T* B, long ldb, char* ws,
+                 bool fail, int* waitflag, int* sticky, hipStream_t s, bool sweep) {
+  const bool notran = (trans == 'N' || trans == 'n');
+  Carve cv;
+  int* perm = reinterpret_cast<int*>(ws + cv.take<int>((size_t)n));
+  T* X = reinterpret_cast<T*>(ws + cv.take<T>((size_t)n * nrhs));
+  unsigned* ctl = reinterpret_cast<unsigned*>(
+      ws + cv.take<unsigned>(sweep ? tri_sweep_ctl_words(n) : tri_chain_ctl_words(n)));
+  int* cerr = reinterpret_cast<int*>(ws + cv.take<int>(1));
+  int* gpairs = reinterpret_cast<int*>(ws + cv.take<int>(perm_pairs_ints(0, n)));
+  T* G = sweep ? reinterpret_cast<T*>(ws + cv.take<T>(tri_sweep_g_elems(n))) : nullptr;
+  const long ldx = n;
+  auto sweep_op = [&](int which) {
+    if (sweep) launch_tri_sweep<T>(which, n, nrhs, A, lda, X, ldx, ctl, G, cerr, s);
+    else launch_tri_chain<T>(which, n, nrhs, A, lda, X, ldx, ctl, cerr, s);
+  };
+  launch_fill_i32(cerr, 1, fail ? 1 : 0, s);
+  if (notran) {
+    launch_swaps_to_perm(n, 0, n, ipiv, 1, perm, s, gpairs);
+    K<T>::gather(n, nrhs, perm, B, ldb, X, ldx, s);  // X = P*B (dgetrs.f:187)
+    sweep_op(0);                                      // L Y = X (dgetrs.f:191)
+    sweep_op(1);                                      // U X = Y (dgetrs.f:197)
+    K<T>::copy(n, nrhs, X, ldx, B, ldb, s, 0, cerr);
+  } else {
+    K<T>::copy(n, nrhs, B, ldb, X, ldx, s);
+    sweep_op(2);                                      // U^T Y = B (dgetrs.f:205)
+    sweep_op(3);                                      // L^T X = Y (dgetrs.f:211)
+    launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s, gpairs);  // dgetrs.f:217
+    K<T>::gather(n, nrhs, perm, X, ldx, B, ldb, s, cerr);
+  }
+  launch_fold_err(cerr, waitflag, sticky, s);
 }
 
 // Solve op(A) X = B with the LU of getrf (SRC/dgetrs.f:181-218).  Device
 // pointers, asynchronous on s.  Two forms of the triangular sweeps:
-//   * one tri_chain launch per sweep (getrs.hip), the default: workgroups in
-//     ticket order with bounded waits for the blocks they depend on.  When a
-//     wait gives up, B is left as passed, bit 0 of the device's sticky
-//     status is set and, when waitflag is given, so is *waitflag (a device
-//     int), so that a synchronous caller can redo the solve;
+//   * getrs_chain, the default: one tri_chain launch per sweep (getrs.hip),
+//     workgroups in ticket order with bounded waits for the blocks they
+//     depend on.  When a wait gives up, B is left as passed, bit 0 of the
+//     device's sticky status is set and, when waitflag is given, so is
+//     *waitflag (a device int), so that a synchronous caller can redo the
+//     solve;
 //   * the blocked form without waits (t_safe, a redone call): the diagonal
 //     blocks of L and U inverted once, every block step two MFMA GEMMs.
 template <class T>
@@ -745,19 +810,9 @@ int getrs_device_ex(char trans, int n, int nrhs, const T* A, long lda, const int
   const int nbs = 256;
   const int nblk = (n + nbs - 1) / nbs;
   char* ws = nullptr;
-  int rc = ws_acquire(c, getrs_ws_bytes<T>(n, nrhs, nbs), s, &ws);
+  const bool sweep = getrs_sweep();
+  int rc = ws_acquire(c, t_safe ? getrs_ws_bytes<T>(n, nrhs, nbs) : getrs_chain_ws_bytes<T>(n, nrhs, sweep), s, &ws);
   if (rc) return rc;
-  Carve cv;
-  int* perm = reinterpret_cast<int*>(ws + cv.take<int>((size_t)n));
-  T* X = reinterpret_cast<T*>(ws + cv.take<T>((size_t)n * nrhs));
-  T* Y = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nbs * nrhs));
-  T* WL = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nbs * nblk * nbs));
-  T* WU = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nbs * nblk * nbs));
-  unsigned* ctl = reinterpret_cast<unsigned*>(ws + cv.take<unsigned>(tri_chain_ctl_words(n)));
-  int* cerr = reinterpret_cast<int*>(ws + cv.take<int>(1));
-  int* gpairs = reinterpret_cast<int*>(ws + cv.take<int>(perm_pairs_ints(0, n)));
-  const long ldx = n, ldy = nbs, ldw = nbs;
-  auto Ap = [&](long r, long col) { return A + r + col * lda; };
 
   PhaseTimer pt;
   pt.on = g_phase_on.load() != 0;
@@ -770,26 +825,21 @@ int getrs_device_ex(char trans, int n, int nrhs, const T* A, long lda, const int
     int fail = g_fail_waits.load();
     while (fail > 0 && !g_fail_waits.compare_exchange_weak(fail, fail - 1)) {
     }
-    launch_fill_i32(cerr, 1, fail > 0 ? 1 : 0, s);
-    if (notran) {
-      launch_swaps_to_perm(n, 0, n, ipiv, 1, perm, s, gpairs);
-      K<T>::gather(n, nrhs, perm, B, ldb, X, ldx, s);                // X = P*B (dgetrs.f:187)
-      launch_tri_chain<T>(0, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // L Y = X (dgetrs.f:191)
-      launch_tri_chain<T>(1, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // U X = Y (dgetrs.f:197)
-      K<T>::copy(n, nrhs, X, ldx, B, ldb, s, 0, cerr);
-    } else {
-      K<T>::copy(n, nrhs, B, ldb, X, ldx, s);
-      launch_tri_chain<T>(2, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // U^T Y = B (dgetrs.f:205)
-      launch_tri_chain<T>(3, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // L^T X = Y (dgetrs.f:211)
-      launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s, gpairs);               // dgetrs.f:217
-      K<T>::gather(n, nrhs, perm, X, ldx, B, ldb, s, cerr);
-    }
-    launch_fold_err(cerr, waitflag, c->status, s);
+    getrs_chain<T>(trans, n, nrhs, A, lda, ipiv, B, ldb, ws, fail > 0, waitflag, c->status, s, sweep);
     if (pt.mark(-1)) fl.push_back(0);
     rc = ws_release(c, s);
     phase_collect(pt, fl);
     return rc;
   }
+  Carve cv;
+  int* perm = reinterpret_cast<int*>(ws + cv.take<int>((size_t)n));
+  T* X = reinterpret_cast<T*>(ws + cv.take<T>((size_t)n * nrhs));
+  T* Y = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nbs * nrhs));
+  T* WL = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nbs * nblk * nbs));
+  T* WU = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nbs * nblk * nbs));
+  int* gpairs = reinterpret_cast<int*>(ws + cv.take<int>(perm_pairs_ints(0, n)));
+  const long ldx = n, ldy = nbs, ldw = nbs;
+  auto Ap = [&](long r, long col) { return A + r + col * lda; };
   K<T>::trtri_blocks(false, true, n, nbs, A, lda, WL, ldw, s);   // inv(L_dd), unit lower
   K<T>::trtri_blocks(true, false, n, nbs, A, lda, WU, ldw, s);   // inv(U_dd), non-unit upper
   if (notran) {
@@ -887,6 +937,9 @@ template int getrs_device_ex<double>(char, int, int, const double*, long, const 
                                      int*);
 template int getrs_device_ex<float>(char, int, int, const float*, long, const int*, float*, long, hipStream_t,
                                     int*);
+template size_t getrs_chain_ws_bytes<double>(int, int, bool);
+template void getrs_chain<double>(char, int, int, const double*, long, const int*, double*, long, char*, bool, int*,
+                                  int*, hipStream_t, bool);
 template int laswp_device<double>(int, double*, long, int, int, int, const int*, int, hipStream_t);
 template int laswp_device<float>(int, float*, long, int, int, int, const int*, int, hipStream_t);
 

@@ -280,7 +280,8 @@ template <class T, int NTH, bool XCD0>
 __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm, long ldt, int j0, int w, int m,
                                                             int* __restrict__ ipiv, int* __restrict__ info,
                                                             unsigned tagbase, u64* __restrict__ xbuf,
-                                                            int* __restrict__ err, int Gp) {
+                                                            int* __restrict__ err, int Gp,
+                                                            unsigned* __restrict__ ctr) {
   constexpr int W = 64;
   constexpr int NW = NTH / 64;
   __shared__ __align__(16) T s_rowk[2][W];
@@ -289,21 +290,36 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
   __shared__ u64 s_wkey[NW];
   __shared__ int s_wrow[NW];
   __shared__ int s_p[2];
+  __shared__ int s_rank;
   const int tid = threadIdx.x;
   int G = gridDim.x;
   int b = blockIdx.x;
   if (XCD0) {
-    // block i runs on XCD i % 8: the blocks of the first panel_xcds(Gp) XCDs
-    // take part, rank r owning rows [r*NTH, r*NTH + NTH); the others leave.
-    // The choice depends on the block index alone, so exactly Gp blocks take
-    // part whatever the dispatch order (the host enables this mode only on a
-    // device of 8 XCDs).
-    const int nx = (int)panel_xcds(Gp);
-    const int x = (int)(blockIdx.x & 7u);
-    const int r = x < nx ? (int)(blockIdx.x >> 3) * nx + x : -1;
-    if (r < 0 || r >= Gp) return;
+    // The blocks that run on XCD 0 (hardware register XCC_ID) take ranks in
+    // arrival order (ctr[0]); the first Gp of them take part, rank r owning
+    // rows [r*NTH, r*NTH + NTH), so the panel owns XCD 0's CUs while the
+    // update leaves them alone; every other block leaves.  Each block then
+    // counts its arrival (ctr[1]); the last to arrive -- every rank decision
+    // is then made -- sets *err when fewer than Gp blocks ran on XCD 0, so
+    // that every wait gives up and the call reports the failure instead of
+    // leaving rows unfactored, and resets both counters for the next launch.
+    if (tid == 0) {
+      int r = -1;
+      if ((__builtin_amdgcn_s_getreg(6164) & 15u) == 0u)  // HW_REG_XCC_ID
+        r = (int)__hip_atomic_fetch_add(&ctr[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned arrived = __hip_atomic_fetch_add(&ctr[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (arrived == gridDim.x - 1) {
+        const unsigned ranks = __hip_atomic_load(&ctr[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (ranks < (unsigned)Gp) st_dev(err, 1);
+        __hip_atomic_store(&ctr[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctr[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      s_rank = (r >= 0 && r < Gp) ? r : -1;
+    }
+    __syncthreads();
+    if (s_rank < 0) return;
     G = Gp;
-    b = r;
+    b = s_rank;
   }
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -635,7 +651,10 @@ int panel_max_width(int h) {
   return 64;
 }
 static size_t tall_offset() { return (sizeof(u64) * 2 * LL_PAR + 255) / 256 * 256; }
-size_t panel_xchg_bytes(int) { return tall_offset() + (sizeof(TallBuf) + 255) / 256 * 256; }
+// the XCD-0 panel's rank and arrival counters (zeroed with the exchange
+// buffer at the start of a factorization; each launch resets its own)
+static size_t rank_offset() { return tall_offset() + (sizeof(TallBuf) + 255) / 256 * 256; }
+size_t panel_xchg_bytes(int) { return rank_offset() + 256; }
 
 // Workgroups the panel kernel uses at height h (0 = a single workgroup).
 int panel_groups(int h) { return h <= 1024 ? 0 : (h + 511) / 512; }
@@ -676,10 +695,12 @@ static int launch_panel_t(T* Tm, long ldt, int j0, int w, int m, int* ipiv, int*
   }
   u64* xb = reinterpret_cast<u64*>(xchg);
   if (mode == PANEL_XCD0)
-    getf2_panel_w8_kernel<T, 512, true><<<(8 / (int)panel_xcds(G)) * G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info,
-                                                                                    tagbase, xb, err, G);
+    getf2_panel_w8_kernel<T, 512, true><<<(8 / (int)panel_xcds(G)) * G, 512, 0, s>>>(
+        Tm, ldt, j0, w, m, ipiv, info, tagbase, xb, err, G,
+        reinterpret_cast<unsigned*>(reinterpret_cast<char*>(xchg) + rank_offset()));
   else
-    getf2_panel_w8_kernel<T, 512, false><<<G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase, xb, err, G);
+    getf2_panel_w8_kernel<T, 512, false><<<G, 512, 0, s>>>(Tm, ldt, j0, w, m, ipiv, info, tagbase, xb, err, G,
+                                                           nullptr);
   return 0;
 }
 

@@ -483,6 +483,17 @@ def ref_idamax(x):
     return r().idamax_(_ri(len(x)), ctypes.c_void_p(_p(x)), _ri(1))
 
 
+def ref_dlange(norm, a):
+    """DLANGE of the reference build (SRC/dlange.f; 'F' through SRC/dlassq.f90)."""
+    a = np.asfortranarray(a, dtype=np.float64)
+    m, n = a.shape
+    work = np.zeros(max(1, m))
+    fn = r().dlange_
+    fn.restype = ctypes.c_double
+    return fn(ctypes.c_char_p(norm.encode()), _ri(m), _ri(n), ctypes.c_void_p(_p(a)), _ri(max(1, m)),
+              ctypes.c_void_p(_p(work)), ctypes.c_size_t(1))
+
+
 def ref_dgeequ(a, radix=False):
     a = np.asfortranarray(a, dtype=np.float64)
     m, n = a.shape

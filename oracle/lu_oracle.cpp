@@ -297,6 +297,76 @@ int getrs(char trans, int n, int nrhs, const T* a, int lda, const int* ipiv, T* 
 }
 
 // DLANGE for NORM in {'M','1'/'O','I'} (SRC/dlange.f:115-185).
+// DLASSQ (SRC/dlassq.f90): scale and sumsq with scale^2*sumsq updated by
+// sum(x_i^2), Blue's three accumulators; constants of SRC/la_constants.f90
+// for FP64 (radix 2, minexponent -1021, maxexponent 1024, digits 53):
+// tsml = 2^-511, tbig = 2^486, ssml = 2^537, sbig = 2^-538.
+inline void lassq(int n, const double* x, double& scale, double& sumsq) {
+  const double tsml = 0x1p-511, tbig = 0x1p486, ssml = 0x1p537, sbig = 0x1p-538;
+  if (std::isnan(scale) || std::isnan(sumsq)) return;
+  if (sumsq == 0.0) scale = 1.0;
+  if (scale == 0.0) {
+    scale = 1.0;
+    sumsq = 0.0;
+  }
+  if (n <= 0) return;
+  bool notbig = true;
+  double asml = 0.0, amed = 0.0, abig = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double ax = std::fabs(x[i]);
+    if (ax > tbig) {
+      abig = abig + (ax * sbig) * (ax * sbig);
+      notbig = false;
+    } else if (ax < tsml) {
+      if (notbig) asml = asml + (ax * ssml) * (ax * ssml);
+    } else {
+      amed = amed + ax * ax;
+    }
+  }
+  if (sumsq > 0.0) {
+    const double ax = scale * std::sqrt(sumsq);
+    if (ax > tbig) {
+      if (scale > 1.0) {
+        scale = scale * sbig;
+        abig = abig + scale * (scale * sumsq);
+      } else {
+        abig = abig + scale * (scale * (sbig * (sbig * sumsq)));
+      }
+    } else if (ax < tsml) {
+      if (notbig) {
+        if (scale < 1.0) {
+          scale = scale * ssml;
+          asml = asml + scale * (scale * sumsq);
+        } else {
+          asml = asml + scale * (scale * (ssml * (ssml * sumsq)));
+        }
+      }
+    } else {
+      amed = amed + scale * (scale * sumsq);
+    }
+  }
+  if (abig > 0.0) {
+    if (amed > 0.0 || std::isnan(amed)) abig = abig + (amed * sbig) * sbig;
+    scale = 1.0 / sbig;
+    sumsq = abig;
+  } else if (asml > 0.0) {
+    if (amed > 0.0 || std::isnan(amed)) {
+      amed = std::sqrt(amed);
+      asml = std::sqrt(asml) / ssml;
+      const double ymin = asml > amed ? amed : asml;
+      const double ymax = asml > amed ? asml : amed;
+      scale = 1.0;
+      sumsq = ymax * ymax * (1.0 + (ymin / ymax) * (ymin / ymax));
+    } else {
+      scale = 1.0 / ssml;
+      sumsq = asml;
+    }
+  } else {
+    scale = 1.0;
+    sumsq = amed;
+  }
+}
+
 template <class T> T lange(char norm, int m, int n, const T* a, int lda) {
   if (std::min(m, n) == 0) return T(0);
   T value = T(0);
@@ -320,6 +390,15 @@ template <class T> T lange(char norm, int m, int n, const T* a, int lda) {
       T temp = work[i];
       if (value < temp || std::isnan(temp)) value = temp;
     }
+  } else if (norm == 'F' || norm == 'f' || norm == 'E' || norm == 'e') {
+    // dlange.f:190-200: DLASSQ over the columns
+    double scale = 0.0, sum = 1.0;
+    std::vector<double> col((size_t)m);
+    for (int j = 1; j <= n; ++j) {
+      for (int i = 1; i <= m; ++i) col[i - 1] = (double)AT(a, i, j, lda);
+      lassq(m, col.data(), scale, sum);
+    }
+    value = (T)(scale * std::sqrt(sum));
   }
   return value;
 }

@@ -90,7 +90,7 @@ def check_matrix(bt, k, lu_all, piv_all, x_all, info_all):
         lur, pr, infor = oracle.sgetrf(a.astype(np.float32))
         lur = lur.astype(np.float64)
     assert info_all[k] == infor, (k, n, info_all[k], infor)
-    assert_ipiv(p, pr, lur)
+    assert_ipiv(p, pr, lur, unit=eps_of(bt.dtype))
     assert oracle.dget01(a, lu, p) * scale < THRESH, (k, n)
     if infor == 0 and bt.nrhs[k] > 0:
         b = bt.b_host(k)

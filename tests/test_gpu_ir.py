@@ -92,7 +92,7 @@ def check_solution(a, b, x, berr, it):
     n = a.shape[0]
     assert it >= 0, it
     assert np.all(berr <= EPS * np.sqrt(n) * (1 + 1e-12)), berr
-    assert oracle.dget08(a, x, b) < max(30.0, 2.0 * np.sqrt(n))
+    assert oracle.dget08(a, x, b) < np.sqrt(n)  # DDRVAB's bar for ITER >= 0 (ddrvab.f:362-365)
     lu, p, _ = oracle.dgetrf(a)
     xr, _ = oracle.dgetrs(lu, p, b)
     kappa = np.linalg.cond(a, np.inf)
@@ -129,29 +129,22 @@ def test_ir_conditioned_1e6(dev, prec):
 
 
 @pytest.mark.parametrize("prec", ["S", "H", "B"])
-def test_ir_conditioned_geometric_contract(dev, prec):
-    """kappa = 1e6 with a geometric spectrum: about half the singular values
-    lie below the FP16/BF16 unit roundoff, so the preconditioned operator has
-    hundreds of outlying eigenvalues and GMRES(50) may not reach the stopping
-    test within its 30 cycles.  The contract is DSGESV's
-    (SRC/dsgesv.f:413-427): either convergence with ITER >= 0, or ITER = -31
-    and the FP64 LU's solution."""
+def test_ir_conditioned_geometric(dev, prec):
+    """kappa = 1e6 with a geometric spectrum (the workload's conditioned
+    variant, here at n = 1024): about half the singular values lie below the
+    FP16/BF16 unit roundoff, so classic refinement stalls and GMRES-IR must
+    converge -- ITER >= 0, no FP64 fallback -- with DSGESV's stopping test,
+    DDRVAB's bar and the forward error against the oracle."""
     n = 1024
     a = conditioned(n, 1e6, 710, mode=3)
     b = la.matrix(3, 712, n, 1)
     x, it, git, info, berr = solve_ir(a, b, prec)
     assert info == 0
-    if prec == "S":
-        check_solution(a, b, x, berr, it)
-    elif it >= 0:
-        check_solution(a, b, x, berr, it)
-    else:
-        assert it == -31 and git > 0
-        lu, p, _ = oracle.dgetrf(a)
-        xr, _ = oracle.dgetrs(lu, p, b)
-        assert oracle.dget02(a, x, b) < 30.0
-        assert np.max(np.abs(x - xr)) <= np.linalg.cond(a, np.inf) * n * EPS * np.max(np.abs(xr))
-    print(f"prec {prec}: iter {it}, gmres {git}")
+    assert it >= 0, (prec, it, git)
+    fwd = check_solution(a, b, x, berr, it)
+    if prec in ("H", "B"):
+        assert git > 0
+    print(f"prec {prec}: iter {it}, gmres {git}, berr {berr.max():.2e}, fwd {fwd:.2e}")
 
 
 def test_ir_fallbacks_and_errors(dev):

@@ -37,8 +37,9 @@ def L():
 
 
 def stored(a, layout):
-    """The buffer LAPACKE is given for the logical matrix a (row- or column-major)."""
-    return np.ascontiguousarray(a) if layout == ROW else np.asfortranarray(a)
+    """A copy of the logical matrix a in the buffer layout LAPACKE is given
+    (row- or column-major); never a view of a, which the call overwrites."""
+    return np.array(a, order="C" if layout == ROW else "F", copy=True)
 
 
 def logical(buf, layout):

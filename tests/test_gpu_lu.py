@@ -2,9 +2,17 @@
 
 Bars (the reference's own test ratios):
   * IPIV identical to the reference DGETRF (integer output).  A divergence is
-    accepted only at a documented near-tie: at the first differing step k the
-    reference's second-best candidate has |L(i,k)| >= 1 - 1e-9 (FP64 factors;
-    FP32 factors use 1 - n * 2^-24, the rounding their entries carry).
+    accepted only at a near-tie, traced both ways at the first differing step
+    k: in the reference's factors the row the GPU chose has a multiplier
+    |L| >= 1 - tol_k, and (when the GPU's factors are given) in the GPU's
+    factors the row the reference chose has |L| >= 1 - tol_k, where tol_k is
+    the rounding two LU implementations may each leave in a candidate of the
+    Schur complement at step k, relative to the pivot:
+        tol_k = 10 * gamma_k * max_{i>=k} (|L||U|)_{ik} / |U_kk|,
+        gamma_k = (k+1) u / (1 - (k+1) u)
+    (the standard componentwise bound |dA^(k)| <= gamma_k |L||U|), with the
+    unit roundoff u of the factoring precision: 2^-53 for FP64 factors,
+    2^-24 for FP32 -- one bar for both.
   * DGET01 ||P*L*U - A||_1/(n ||A||_1 eps) < 30 (TESTING/LIN/dget01.f, threshold
     TESTING/dtest.in:13), DGET02 ||B - A X||_1/(||A||_1 ||X||_1 eps) < 30.
   * GEMM: DBLAT3's componentwise ratio |ct - cc|/(eps*g) < 16
@@ -38,13 +46,49 @@ def near_tie(lu_ref, k):
     return 1.0 - col.max()
 
 
-def assert_ipiv(ipiv_gpu, ipiv_ref, lu_ref, tol=1e-9):
-    """tol: the near-tie scale of the factoring precision, 1e-9 for FP64;
-    FP32 factors carry rounding of order n * 2^-24 and pass that."""
+def final_row(ipiv, k, r):
+    """Row of the final factors holding the content that sat in row r when
+    step k began (0-based), after step k's and every later interchange."""
+    p = int(ipiv[k]) - 1
+    pos = p if r == k else (k if r == p else r)
+    for j in range(k + 1, len(ipiv)):
+        q = int(ipiv[j]) - 1
+        if pos == j:
+            pos = q
+        elif pos == q:
+            pos = j
+    return pos
+
+
+def tie_tolerance(lu, k, unit):
+    """10 * gamma_k * max_{i>=k} (|L||U|)_{ik} / |U_kk| from factors lu."""
+    lu = np.asarray(lu, dtype=np.float64)
+    ukk = abs(float(lu[k, k]))
+    if ukk == 0.0:
+        return np.inf
+    lcol = np.concatenate(([1.0], np.abs(lu[k + 1:, k])))  # column k of L, rows k.. (unit diagonal)
+    g = np.abs(lu[k:, :k]) @ np.abs(lu[:k, k]) + lcol * ukk
+    gam = (k + 1) * unit / (1.0 - (k + 1) * unit)
+    return 10.0 * gam * float(g.max()) / ukk
+
+
+def assert_ipiv(ipiv_gpu, ipiv_ref, lu_ref, lu_gpu=None, unit=2.0 ** -53):
+    """IPIV equal, or a near-tie at the first differing step traced in the
+    reference's factors and, with lu_gpu, in the GPU's (see the module
+    docstring).  unit: unit roundoff of the factoring precision."""
     k = first_mismatch(ipiv_gpu, ipiv_ref)
-    if k >= 0:
-        gap = near_tie(lu_ref, k)
-        assert gap < tol, f"IPIV differs at step {k} without a near-tie (gap {gap:.3e})"
+    if k < 0:
+        return
+    tol = tie_tolerance(np.asarray(lu_ref, dtype=np.float64), k, unit)
+    r_gpu, r_ref = int(ipiv_gpu[k]) - 1, int(ipiv_ref[k]) - 1
+    gap_ref = 1.0 - abs(float(lu_ref[final_row(ipiv_ref, k, r_gpu), k]))
+    msg = f"IPIV differs at step {k}: rows {r_gpu + 1} (GPU) / {r_ref + 1} (reference)"
+    assert gap_ref <= tol, f"{msg}; in the reference's factors the GPU's row has gap {gap_ref:.3e} > {tol:.3e}"
+    if lu_gpu is not None:
+        gap_gpu = 1.0 - abs(float(lu_gpu[final_row(ipiv_gpu, k, r_ref), k]))
+        assert gap_gpu <= tol, f"{msg}; in the GPU's factors the reference's row has gap {gap_gpu:.3e} > {tol:.3e}"
+    print(f"near-tie at step {k}: gaps {gap_ref:.3e} (reference factors)"
+          + (f", {gap_gpu:.3e} (GPU factors)" if lu_gpu is not None else "") + f" <= {tol:.3e}")
 
 
 def gpu_getrf(a, dtype=np.float64):
@@ -101,7 +145,7 @@ def test_dgetrf_square_parity(dev, n):
     lu_r, p_r, info_r = oracle.dgetrf(a)
     lu, p, info = gpu_getrf(a)
     assert info == info_r == 0
-    assert_ipiv(p, p_r, lu_r)
+    assert_ipiv(p, p_r, lu_r, lu)
     assert oracle.dget01(a, lu, p) < THRESH
 
 
@@ -111,7 +155,7 @@ def test_dgetrf_rect_parity(dev, m, n):
     lu_r, p_r, info_r = oracle.dgetrf(a)
     lu, p, info = gpu_getrf(a)
     assert info == info_r
-    assert_ipiv(p, p_r, lu_r)
+    assert_ipiv(p, p_r, lu_r, lu)
     assert oracle.dget01(a, lu, p) < THRESH
 
 
@@ -127,7 +171,7 @@ def test_dgetrf_block_size_sweep(dev, nb, leaf):
     finally:
         la.set_tuning(0, 0)
     assert info == 0
-    assert_ipiv(p, p_r, lu_r)
+    assert_ipiv(p, p_r, lu_r, lu)
     assert oracle.dget01(a, lu, p) < THRESH
 
 
@@ -149,7 +193,7 @@ def test_dgetrf_zero_columns_info(dev, kind):
     lu, p, info = gpu_getrf(a)
     assert info_r == izero
     assert info == izero
-    assert_ipiv(p, p_r, lu_r)
+    assert_ipiv(p, p_r, lu_r, lu)
 
 
 @pytest.mark.parametrize("trans", ["N", "T"])
@@ -220,9 +264,7 @@ def test_sgetrf_parity(dev, n):
     lu_r, p_r, _ = oracle.sgetrf(a)
     lu, p, info = gpu_getrf(a, np.float32)
     assert info == 0
-    k = first_mismatch(p, p_r)
-    if k >= 0:
-        assert near_tie(lu_r.astype(np.float64), k) < 1e-5
+    assert_ipiv(p, p_r, lu_r.astype(np.float64), lu.astype(np.float64), unit=2.0 ** -24)
     # SGET01-style ratio in single precision
     l = np.tril(lu.astype(np.float64), -1) + np.eye(n)
     u = np.triu(lu.astype(np.float64))

@@ -145,7 +145,7 @@ def test_dchkge_matrix_kinds(dev, imat, n):
     lu_r, p_r, info_r = oracle.dgetrf(a)
     lu, p, info = gpu_getrf(a)
     assert info == info_r
-    assert_ipiv(p, p_r, lu_r)
+    assert_ipiv(p, p_r, lu_r, lu)
     assert oracle.dget01(a, lu, p) < THRESH
     d_lu, d_p, d_b = to_dev(lu), to_dev(p.reshape(-1, 1).astype(np.int32)), to_dev(b)
     assert la.dgetrs("N", n, 2, d_lu, n, d_p, d_b, n) == 0

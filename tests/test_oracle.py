@@ -254,6 +254,37 @@ def test_dget_ratios_closed_form():
     assert oracle.dlange("1", a) == 6.0 and oracle.dlange("I", a) == 6.0 and oracle.dlange("M", a) == 4.0
 
 
+def test_dlange_frobenius():
+    """DLANGE('F') (SRC/dlange.f:190-200, DLASSQ of SRC/dlassq.f90): closed
+    forms -- sqrt(m n) for a matrix of ones, exact for a scaled 3-4-5
+    triangle at magnitudes where the plain sum of squares over- or
+    underflows (Blue's big and small accumulators) -- and, where the
+    reference build exists, bitwise equal to it on random, huge, tiny and
+    mixed matrices."""
+    for m, n in ((1, 1), (3, 7), (64, 5)):
+        assert oracle.dlange("F", np.ones((m, n))) == pytest.approx(np.sqrt(m * n), rel=1e-15)
+    for s in (1.0, 1e300, 1e-300, 2.0 ** -1070):
+        a = np.asfortranarray([[3.0 * s], [4.0 * s]])
+        assert oracle.dlange("F", a) == pytest.approx(5.0 * s, rel=1e-15), s
+        assert oracle.dlange("E", a) == oracle.dlange("F", a)
+    # the unscaled sum of squares would give 0 and inf at these magnitudes
+    assert np.sqrt(np.sum(np.array([3e-300, 4e-300]) ** 2)) == 0.0
+    assert np.isinf(np.sqrt(np.sum(np.array([3e300, 4e300]) ** 2)))
+    if oracle.ref_available():
+        rng = np.random.default_rng(7)
+        mats = [rng.standard_normal((37, 20)), rng.standard_normal((300, 3)) * 1e250,
+                rng.standard_normal((5, 9)) * 1e-250, np.zeros((4, 4))]
+        mixed = rng.standard_normal((30, 30))
+        mixed[::3] *= 1e200
+        mixed[1::3] *= 1e-200
+        mats.append(mixed)
+        for a in mats:
+            a = np.asfortranarray(a)
+            assert oracle.dlange("F", a) == oracle.ref_dlange("F", a)
+            for norm in "M1OI":
+                assert oracle.dlange(norm, a) == oracle.ref_dlange(norm, a)
+
+
 def test_dchkeq_closed_form():
     """TESTING/LIN/dchkeq.f:107-173: A(i,j) = 10^(i+j) (-1)^(i+j)."""
     eps = 2.0 ** -52
