@@ -395,8 +395,9 @@ __device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, int vo
 // 128 x 128 tile at a time, each wave 32 x 64 = 2 x 4 MFMA blocks (32 FP64
 // accumulators per lane), which leaves registers for a second set: while a
 // tile's k loop runs, the C elements of the workgroup's next tile are read
-// into that set, one block per k step, issued after the step's A/B loads so
-// that waiting for those never waits for C.  In the tile's last k step the
+// into that set, one block per k step, issued once the step's A/B slice has
+// arrived, so that they have a whole step to arrive before the next step's
+// wait for its own slice covers them.  In the tile's last k step the
 // A/B loads fetch the next tile's first k slice.  A tile's end is then its C
 // stores and a register copy; no read of C or of the first slice is waited
 // for between tiles.  BK = 32 per LDS stage (137 KB of LDS for FP64, one
@@ -503,6 +504,11 @@ __global__ __launch_bounds__(512) void update_pipe_kernel(int K, const T* __rest
 #pragma unroll
     for (int j = 0; j < 4; ++j) read_c(acc[i][j], row0, col0, i, j);
   store_tile(0);
+  // the first tile's C has arrived before the loop: otherwise the
+  // compiler's count of outstanding loads at the loop head keeps the
+  // prologue's, and every k step waits for all of its loads (the next
+  // slice included) before its first MFMA
+  wait_vm();
   __syncthreads();
   int it = 0;
 #pragma nounroll
@@ -517,12 +523,6 @@ __global__ __launch_bounds__(512) void update_pipe_kernel(int K, const T* __rest
       const bool nxt = kt + 1 < nkt || more;
       if (kt + 1 < nkt) load_tile(row0, col0, kt + 1);
       else if (more) load_tile(row1, col1, 0);
-      if (more) {
-        // the next tile's C, group g read at step g * nkt / NG
-#pragma unroll
-        for (int g = 0; g < NG; ++g)
-          if ((g * nkt) / NG == kt) read_c(cb[g / 4][g % 4], row1, col1, g / 4, g % 4);
-      }
       const T* as = As[cur];
       const T* bs = Bs[cur];
       // operand fragments one 4-deep k step ahead of the MFMAs that use
@@ -546,6 +546,14 @@ __global__ __launch_bounds__(512) void update_pipe_kernel(int K, const T* __rest
         __builtin_amdgcn_sched_barrier(0);
       }
       if (nxt) store_tile(cur ^ 1);
+      // the next tile's C, group g read at step g * nkt / NG, after this
+      // step's A/B slice has been waited for: the next step's wait for its
+      // own (younger) slice then covers these, a whole step later
+      if (more) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+          if ((g * nkt) / NG == kt) read_c(cb[g / 4][g % 4], row1, col1, g / 4, g % 4);
+      }
       __syncthreads();
     }
     // this tile's results (C was read into the accumulators)
@@ -758,13 +766,10 @@ static int update_small_tiles() {
 
 // Large updates on the software-pipelined kernel (LAPACK_AMD_UPD_PIPE=1) or
 // the 256x128 one-tile-per-workgroup kernel (0).
+// (read at every launch, so that a process may switch it)
 static int update_pipe() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = std::getenv("LAPACK_AMD_UPD_PIPE");
-    v = (e && *e) ? std::atoi(e) : 0;
-  }
-  return v;
+  const char* e = std::getenv("LAPACK_AMD_UPD_PIPE");
+  return (e && *e) ? std::atoi(e) : 0;
 }
 
 template <class T>
@@ -786,7 +791,10 @@ void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long
     const int mi = tmp * 128, ni = tnp * 128;
     const int chunk = (tmp * tnp + nx - 1) / nx;
     int per = std::min(chunk, 32);
-    if (nskip > 0 && 2.0 * m * (double)n * k < update_persist_flops()) per = std::min(per, sizeof(T) == 4 ? update_cus_per_xcd_s() : update_cus_per_xcd());
+    // beside a panel the persistent kernel always leaves CUs of every XCD
+    // free: the panel's own launches there would otherwise wait for the
+    // whole update
+    if (nskip > 0) per = std::min(per, sizeof(T) == 4 ? update_cus_per_xcd_s() : update_cus_per_xcd());
     update_pipe_kernel<T><<<dim3(per * NXCD), 512, 0, s>>>(k, A, lda, B, ldb, C, ldc, tmp, tnp, chunk, nx);
     if (ni < n) {
       const int tm1 = (m + BM - 1) / BM, tn1 = (n - ni + BN - 1) / BN;

@@ -202,14 +202,6 @@ int launch_adjoint_batched(int grid, int batch, int nmax, const int* nv, const i
 // ctl: tri_chain_ctl_words(n) unsigned of scratch (cleared by the launcher);
 // err: set to 1 if a wait timed out.
 size_t tri_chain_ctl_words(int n);
-// The same sweep as a chain workgroup plus a helper per 64-row block
-// (getrs.hip): ctl tri_sweep_ctl_words(n) unsigned, G tri_sweep_g_elems(n)
-// elements of scratch; err as above.
-size_t tri_sweep_ctl_words(int n);
-size_t tri_sweep_g_elems(int n);
-template <class T>
-void launch_tri_sweep(int which, int n, int nrhs, const T* A, long lda, T* X, long ldx, unsigned* ctl, T* G,
-                      int* err, hipStream_t s);
 template <class T>
 void launch_tri_chain(int which, int n, int nrhs, const T* A, long lda, T* X, long ldx, unsigned* ctl, int* err,
                       hipStream_t s);

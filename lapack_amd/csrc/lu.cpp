@@ -57,6 +57,10 @@ struct DevCtx {
   size_t mp_bytes = 0;
   int* status = nullptr;         // sticky device word: bit 0 = a bounded wait gave up
   bool xcd8 = false;             // 8 XCDs of >= 32 CUs: the panel may own XCD 0
+  // flag base of the next panel launch: grows by 64 per launch across calls,
+  // so that an exchange word left from an earlier launch (in memory or in an
+  // XCD's L2) never carries a current flag
+  unsigned leaf_tag = 0;
   std::mutex mu;     // guards ws and the launch sequence of one call
   std::mutex io_mu;  // guards io across a whole synchronous call
   std::mutex mp_mu;  // guards mp across a whole mixed-precision solve
@@ -544,7 +548,9 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
   auto endm = [&](PhaseTimer& pt, std::vector<double>& fl) { mark(pt, fl, -1, 0); };
   auto Tp = [&](long r, long col) { return Tm + r + col * ldt; };
 
-  // the panel's flag-in-data exchange needs a zeroed buffer per factorization
+  // the panel's flag-in-data exchange needs a zeroed buffer per factorization;
+  // the flag base starts over only long before it could wrap
+  if (c->leaf_tag > 0xF0000000u) c->leaf_tag = 0;
   launch_fill_u32(reinterpret_cast<unsigned*>(xchg), panel_xchg_bytes(0) / sizeof(unsigned), 0u, s);
   launch_fill_i32(err, 1, fail > 0 ? 1 : 0, s);
   launch_fill_i32(info, 1, 0, s);
@@ -552,7 +558,6 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
   K<T>::transpose(m, n, A, lda, Tm, ldt, s);  // T(c, r) = A(r, c)
   endm(pa, fla);
 
-  int leafno = 0;
   // Panel j: A(j:m, j:j+jb) = T rows j..j+jb, T columns j..m; the DGETRF2
   // recursion flattened to leaves, with in-panel interchanges, DTRSM and
   // DGEMM between leaves (buffers of slot 1, stream st).
@@ -563,8 +568,8 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
       const int h = m - jj;
       const int w = std::min({leaf, panel_max_width(h), j + jb - jj});
       mark(pt, fl, LA_PHASE_PANEL, 0);
-      ++leafno;
-      int r = K<T>::panel(Tm, ldt, jj, w, m, ipiv, info, 64u * (unsigned)leafno, xchg, err, mode, st);
+      c->leaf_tag += 64u;
+      int r = K<T>::panel(Tm, ldt, jj, w, m, ipiv, info, c->leaf_tag, xchg, err, mode, st);
       endm(pt, fl);
       if (r) return r;
       const int rest = j + jb - jj - w;
@@ -711,26 +716,15 @@ int getrf_device(int m, int n, T* A, long lda, int* ipiv, int* info, hipStream_t
 
 // --------------------------------------------------------------- GETRS --
 
-// DGETRS's triangular sweeps on tri_sweep (LAPACK_AMD_GETRS_SWEEP=1) or on
-// tri_chain (0, the default).
-static bool getrs_sweep() {
-  static int v = -1;
-  if (v < 0) v = env_int("LAPACK_AMD_GETRS_SWEEP", 0);
-  return v != 0;
-}
-
-// Workspace of the chain form of DGETRS (getrs_chain); sweep: the
-// chain-and-helpers sweep kernel (its G tiles: nblk * 3 * 64 * 64 elements)
-// rather than one workgroup per block.
+// Workspace of the chain form of DGETRS (getrs_chain).
 template <class T>
-size_t getrs_chain_ws_bytes(int n, int nrhs, bool sweep) {
+size_t getrs_chain_ws_bytes(int n, int nrhs) {
   Carve cv;
   cv.take<int>((size_t)n);
   cv.take<T>((size_t)n * nrhs);
-  cv.take<unsigned>(sweep ? tri_sweep_ctl_words(n) : tri_chain_ctl_words(n));
+  cv.take<unsigned>(tri_chain_ctl_words(n));
   cv.take<int>(1);
   cv.take<int>(perm_pairs_ints(0, n));
-  if (sweep) cv.take<T>(tri_sweep_g_elems(n));
   return cv.off;
 }
 
@@ -744,46 +738,38 @@ size_t getrs_ws_bytes(int n, int nrhs, int nbs) {
   cv.take<T>((size_t)nbs * (size_t)((n + nbs - 1) / nbs) * nbs);
   cv.take<T>((size_t)nbs * (size_t)((n + nbs - 1) / nbs) * nbs);
   cv.take<int>(perm_pairs_ints(0, n));
-  return std::max(cv.off, getrs_chain_ws_bytes<T>(n, nrhs, true));
+  return std::max(cv.off, getrs_chain_ws_bytes<T>(n, nrhs));
 }
 
 // The chain form of DGETRS (SRC/dgetrs.f:181-218) on the device workspace
-// ws (getrs_chain_ws_bytes(n, nrhs, sweep) bytes, 256-byte aligned): the row
-// permutation, then one launch per triangular sweep -- tri_sweep (a chain
-// workgroup and a helper per 64-row block) when sweep, else tri_chain (a
-// workgroup per block in ticket order; getrs.hip).  When a wait gives up, B
-// is left as passed and the failure is folded into *waitflag and *sticky
-// (device ints; either may be null).  fail: start with the wait-error word
-// set (the test hook).  Enqueued on s; allocates nothing.
+// ws (getrs_chain_ws_bytes(n, nrhs) bytes, 256-byte aligned): the row
+// permutation, then one tri_chain launch per triangular sweep (getrs.hip).
+// When a wait gives up, B is left as passed and the failure is folded into
+// *waitflag and *sticky (device ints; either may be null).  fail: start with
+// the wait-error word set (the test hook).  Enqueued on s; allocates nothing.
 template <class T>
 void getrs_chain(char trans, int n, int nrhs, const T* A, long lda, const int* ipiv, T* B, long ldb, char* ws,
-                 bool fail, int* waitflag, int* sticky, hipStream_t s, bool sweep) {
+                 bool fail, int* waitflag, int* sticky, hipStream_t s) {
   const bool notran = (trans == 'N' || trans == 'n');
   Carve cv;
   int* perm = reinterpret_cast<int*>(ws + cv.take<int>((size_t)n));
   T* X = reinterpret_cast<T*>(ws + cv.take<T>((size_t)n * nrhs));
-  unsigned* ctl = reinterpret_cast<unsigned*>(
-      ws + cv.take<unsigned>(sweep ? tri_sweep_ctl_words(n) : tri_chain_ctl_words(n)));
+  unsigned* ctl = reinterpret_cast<unsigned*>(ws + cv.take<unsigned>(tri_chain_ctl_words(n)));
   int* cerr = reinterpret_cast<int*>(ws + cv.take<int>(1));
   int* gpairs = reinterpret_cast<int*>(ws + cv.take<int>(perm_pairs_ints(0, n)));
-  T* G = sweep ? reinterpret_cast<T*>(ws + cv.take<T>(tri_sweep_g_elems(n))) : nullptr;
   const long ldx = n;
-  auto sweep_op = [&](int which) {
-    if (sweep) launch_tri_sweep<T>(which, n, nrhs, A, lda, X, ldx, ctl, G, cerr, s);
-    else launch_tri_chain<T>(which, n, nrhs, A, lda, X, ldx, ctl, cerr, s);
-  };
   launch_fill_i32(cerr, 1, fail ? 1 : 0, s);
   if (notran) {
     launch_swaps_to_perm(n, 0, n, ipiv, 1, perm, s, gpairs);
-    K<T>::gather(n, nrhs, perm, B, ldb, X, ldx, s);  // X = P*B (dgetrs.f:187)
-    sweep_op(0);                                      // L Y = X (dgetrs.f:191)
-    sweep_op(1);                                      // U X = Y (dgetrs.f:197)
+    K<T>::gather(n, nrhs, perm, B, ldb, X, ldx, s);                // X = P*B (dgetrs.f:187)
+    launch_tri_chain<T>(0, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // L Y = X (dgetrs.f:191)
+    launch_tri_chain<T>(1, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // U X = Y (dgetrs.f:197)
     K<T>::copy(n, nrhs, X, ldx, B, ldb, s, 0, cerr);
   } else {
     K<T>::copy(n, nrhs, B, ldb, X, ldx, s);
-    sweep_op(2);                                      // U^T Y = B (dgetrs.f:205)
-    sweep_op(3);                                      // L^T X = Y (dgetrs.f:211)
-    launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s, gpairs);  // dgetrs.f:217
+    launch_tri_chain<T>(2, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // U^T Y = B (dgetrs.f:205)
+    launch_tri_chain<T>(3, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // L^T X = Y (dgetrs.f:211)
+    launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s, gpairs);       // dgetrs.f:217
     K<T>::gather(n, nrhs, perm, X, ldx, B, ldb, s, cerr);
   }
   launch_fold_err(cerr, waitflag, sticky, s);
@@ -810,8 +796,7 @@ int getrs_device_ex(char trans, int n, int nrhs, const T* A, long lda, const int
   const int nbs = 256;
   const int nblk = (n + nbs - 1) / nbs;
   char* ws = nullptr;
-  const bool sweep = getrs_sweep();
-  int rc = ws_acquire(c, t_safe ? getrs_ws_bytes<T>(n, nrhs, nbs) : getrs_chain_ws_bytes<T>(n, nrhs, sweep), s, &ws);
+  int rc = ws_acquire(c, t_safe ? getrs_ws_bytes<T>(n, nrhs, nbs) : getrs_chain_ws_bytes<T>(n, nrhs), s, &ws);
   if (rc) return rc;
 
   PhaseTimer pt;
@@ -825,7 +810,7 @@ int getrs_device_ex(char trans, int n, int nrhs, const T* A, long lda, const int
     int fail = g_fail_waits.load();
     while (fail > 0 && !g_fail_waits.compare_exchange_weak(fail, fail - 1)) {
     }
-    getrs_chain<T>(trans, n, nrhs, A, lda, ipiv, B, ldb, ws, fail > 0, waitflag, c->status, s, sweep);
+    getrs_chain<T>(trans, n, nrhs, A, lda, ipiv, B, ldb, ws, fail > 0, waitflag, c->status, s);
     if (pt.mark(-1)) fl.push_back(0);
     rc = ws_release(c, s);
     phase_collect(pt, fl);
@@ -937,9 +922,9 @@ template int getrs_device_ex<double>(char, int, int, const double*, long, const 
                                      int*);
 template int getrs_device_ex<float>(char, int, int, const float*, long, const int*, float*, long, hipStream_t,
                                     int*);
-template size_t getrs_chain_ws_bytes<double>(int, int, bool);
+template size_t getrs_chain_ws_bytes<double>(int, int);
 template void getrs_chain<double>(char, int, int, const double*, long, const int*, double*, long, char*, bool, int*,
-                                  int*, hipStream_t, bool);
+                                  int*, hipStream_t);
 template int laswp_device<double>(int, double*, long, int, int, int, const int*, int, hipStream_t);
 template int laswp_device<float>(int, float*, long, int, int, int, const int*, int, hipStream_t);
 
