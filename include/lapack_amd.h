@@ -182,6 +182,20 @@ char lapack_amd_dlaqge(int m, int n, double* a, int lda, const double* r, const 
 int lapack_amd_dgetrf_async(int m, int n, double* d_a, int lda, int* d_ipiv, int* d_info, void* stream);
 int lapack_amd_dgetrs_async(char trans, int n, int nrhs, const double* d_a, int lda, const int* d_ipiv,
                             double* d_b, int ldb, void* stream);
+/* No counterpart in the reference: DGETRS without any device allocation, for
+ * callers that cannot allocate (code under HIP graph capture, serving loops
+ * or harnesses that forbid hipMalloc).  Same arguments, checks and kernels as
+ * lapack_amd_dgetrs_async, plus d_info (a device int): 0, or LA_ERR_SYNC when
+ * a bounded wait gave up (B then left as passed).  The workspace is a static
+ * device array of lapack_amd_noalloc_scratch_bytes() (16 MiB) per GPU; a call
+ * needing more than that (lapack_amd_dgetrs_noalloc_bytes(n, nrhs): n ints,
+ * n * nrhs doubles and a few KB, about 2.2 MB at n = 16384, nrhs = 16;
+ * n * nrhs up to about 2e6) returns LA_ERR_LIMIT.  Calls that overlap on
+ * one GPU share the array and race; order them on one stream. */
+int lapack_amd_dgetrs_noalloc_async(char trans, int n, int nrhs, const double* d_a, int lda, const int* d_ipiv,
+                                    double* d_b, int ldb, int* d_info, void* stream);
+size_t lapack_amd_dgetrs_noalloc_bytes(int n, int nrhs);
+size_t lapack_amd_noalloc_scratch_bytes(void);
 int lapack_amd_sgetrf_async(int m, int n, float* d_a, int lda, int* d_ipiv, int* d_info, void* stream);
 int lapack_amd_sgetrs_async(char trans, int n, int nrhs, const float* d_a, int lda, const int* d_ipiv,
                             float* d_b, int ldb, void* stream);

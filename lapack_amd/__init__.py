@@ -69,6 +69,9 @@ def _declare(L):
         "lapack_amd_dgetrf_async": (i, [i, i, vp, i, vp, vp, vp]),
         "lapack_amd_sgetrf_async": (i, [i, i, vp, i, vp, vp, vp]),
         "lapack_amd_dgetrs_async": (i, [ch, i, i, vp, i, vp, vp, i, vp]),
+        "lapack_amd_dgetrs_noalloc_async": (i, [ch, i, i, vp, i, vp, vp, i, vp, vp]),
+        "lapack_amd_dgetrs_noalloc_bytes": (ctypes.c_size_t, [i, i]),
+        "lapack_amd_noalloc_scratch_bytes": (ctypes.c_size_t, []),
         "lapack_amd_sgetrs_async": (i, [ch, i, i, vp, i, vp, vp, i, vp]),
         "lapack_amd_dlaswp_async": (i, [i, vp, i, i, i, vp, i, vp]),
         "lapack_amd_dgemm_async": (i, [ch, ch, i, i, i, d, vp, i, vp, i, d, vp, i, vp]),
@@ -251,6 +254,13 @@ def sgetrf_async(m, n, d_a, lda, d_ipiv, d_info, stream=None):
 def dgetrs_async(trans, n, nrhs, d_a, lda, d_ipiv, d_b, ldb, stream=None):
     return lib().lapack_amd_dgetrs_async(_c(trans), n, nrhs, _ptr(d_a), lda, _ptr(d_ipiv), _ptr(d_b), ldb,
                                          _stream(stream))
+
+
+def dgetrs_noalloc_async(trans, n, nrhs, d_a, lda, d_ipiv, d_b, ldb, d_info, stream=None):
+    """DGETRS with its workspace in the library's static device array (no
+    device allocation); d_info receives 0 or LA_ERR_SYNC."""
+    return lib().lapack_amd_dgetrs_noalloc_async(_c(trans), n, nrhs, _ptr(d_a), lda, _ptr(d_ipiv), _ptr(d_b), ldb,
+                                                 _ptr(d_info), _stream(stream))
 
 
 def sgetrs_async(trans, n, nrhs, d_a, lda, d_ipiv, d_b, ldb, stream=None):
