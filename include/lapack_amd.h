@@ -189,8 +189,8 @@ int lapack_amd_dgetrs_async(char trans, int n, int nrhs, const double* d_a, int 
  * a bounded wait gave up (B then left as passed).  The workspace is a static
  * device array of lapack_amd_noalloc_scratch_bytes() (16 MiB) per GPU; a call
  * needing more than that (lapack_amd_dgetrs_noalloc_bytes(n, nrhs): n ints,
- * n * nrhs doubles and a few KB, about 2.2 MB at n = 16384, nrhs = 16;
- * n * nrhs up to about 2e6) returns LA_ERR_LIMIT.  Calls that overlap on
+ * 2 n * nrhs doubles and a few KB, about 4.3 MB at n = 16384, nrhs = 16;
+ * n * nrhs up to about 1e6) returns LA_ERR_LIMIT.  Calls that overlap on
  * one GPU share the array and race; order them on one stream. */
 int lapack_amd_dgetrs_noalloc_async(char trans, int n, int nrhs, const double* d_a, int lda, const int* d_ipiv,
                                     double* d_b, int ldb, int* d_info, void* stream);

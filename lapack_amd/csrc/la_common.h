@@ -198,13 +198,15 @@ int launch_adjoint_batched(int grid, int batch, int nmax, const int* nv, const i
                            int accumulate, const int* order, unsigned* counter, hipStream_t s);
 
 // Triangular sweeps of DGETRS in one launch each (getrs.hip).  which: 0 = L
-// (unit lower), 1 = U, 2 = U^T, 3 = L^T; X is n x nrhs (ldx), solved in place.
-// ctl: tri_chain_ctl_words(n) unsigned of scratch (cleared by the launcher);
-// err: set to 1 if a wait timed out.
-size_t tri_chain_ctl_words(int n);
+// (unit lower), 1 = U, 2 = U^T, 3 = L^T; the right-hand sides Xin (n x nrhs,
+// ldxi) are read, the solution written to X (ldx; a different buffer, whose
+// previous contents are overwritten from the start).  ctl:
+// tri_chain_ctl_words(nrhs) unsigned of scratch (cleared by the launcher);
+// err: set to 1 if a wait timed out (X then incomplete).
+size_t tri_chain_ctl_words(int nrhs);
 template <class T>
-void launch_tri_chain(int which, int n, int nrhs, const T* A, long lda, T* X, long ldx, unsigned* ctl, int* err,
-                      hipStream_t s);
+void launch_tri_chain(int which, int n, int nrhs, const T* A, long lda, const T* Xin, long ldxi, T* X, long ldx,
+                      unsigned* ctl, int* err, hipStream_t s);
 
 // Rank-k outer product C <- alpha*A*B^T + beta*C for k <= rankk_max()
 // (adjoint.hip), the Abar = -Bbar X^T step of the solve's adjoint.

@@ -722,7 +722,8 @@ size_t getrs_chain_ws_bytes(int n, int nrhs) {
   Carve cv;
   cv.take<int>((size_t)n);
   cv.take<T>((size_t)n * nrhs);
-  cv.take<unsigned>(tri_chain_ctl_words(n));
+  cv.take<T>((size_t)n * nrhs);
+  cv.take<unsigned>(tri_chain_ctl_words(nrhs));
   cv.take<int>(1);
   cv.take<int>(perm_pairs_ints(0, n));
   return cv.off;
@@ -754,22 +755,23 @@ void getrs_chain(char trans, int n, int nrhs, const T* A, long lda, const int* i
   Carve cv;
   int* perm = reinterpret_cast<int*>(ws + cv.take<int>((size_t)n));
   T* X = reinterpret_cast<T*>(ws + cv.take<T>((size_t)n * nrhs));
-  unsigned* ctl = reinterpret_cast<unsigned*>(ws + cv.take<unsigned>(tri_chain_ctl_words(n)));
+  T* Y = reinterpret_cast<T*>(ws + cv.take<T>((size_t)n * nrhs));
+  unsigned* ctl = reinterpret_cast<unsigned*>(ws + cv.take<unsigned>(tri_chain_ctl_words(nrhs)));
   int* cerr = reinterpret_cast<int*>(ws + cv.take<int>(1));
   int* gpairs = reinterpret_cast<int*>(ws + cv.take<int>(perm_pairs_ints(0, n)));
   const long ldx = n;
   launch_fill_i32(cerr, 1, fail ? 1 : 0, s);
   if (notran) {
     launch_swaps_to_perm(n, 0, n, ipiv, 1, perm, s, gpairs);
-    K<T>::gather(n, nrhs, perm, B, ldb, X, ldx, s);                // X = P*B (dgetrs.f:187)
-    launch_tri_chain<T>(0, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // L Y = X (dgetrs.f:191)
-    launch_tri_chain<T>(1, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // U X = Y (dgetrs.f:197)
+    K<T>::gather(n, nrhs, perm, B, ldb, X, ldx, s);                         // X = P*B (dgetrs.f:187)
+    launch_tri_chain<T>(0, n, nrhs, A, lda, X, ldx, Y, ldx, ctl, cerr, s);  // L Y = X (dgetrs.f:191)
+    launch_tri_chain<T>(1, n, nrhs, A, lda, Y, ldx, X, ldx, ctl, cerr, s);  // U X = Y (dgetrs.f:197)
     K<T>::copy(n, nrhs, X, ldx, B, ldb, s, 0, cerr);
   } else {
     K<T>::copy(n, nrhs, B, ldb, X, ldx, s);
-    launch_tri_chain<T>(2, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // U^T Y = B (dgetrs.f:205)
-    launch_tri_chain<T>(3, n, nrhs, A, lda, X, ldx, ctl, cerr, s);  // L^T X = Y (dgetrs.f:211)
-    launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s, gpairs);       // dgetrs.f:217
+    launch_tri_chain<T>(2, n, nrhs, A, lda, X, ldx, Y, ldx, ctl, cerr, s);  // U^T Y = B (dgetrs.f:205)
+    launch_tri_chain<T>(3, n, nrhs, A, lda, Y, ldx, X, ldx, ctl, cerr, s);  // L^T X = Y (dgetrs.f:211)
+    launch_swaps_to_perm(n, 0, n, ipiv, -1, perm, s, gpairs);                // dgetrs.f:217
     K<T>::gather(n, nrhs, perm, X, ldx, B, ldb, s, cerr);
   }
   launch_fold_err(cerr, waitflag, sticky, s);

@@ -831,13 +831,13 @@ int gecon_device(bool onenrm, int n, const double* AF, long ldaf, double anorm, 
   if (!ctx) return LA_ERR_NODEVICE;
   std::lock_guard<std::mutex> g(mp_mutex(ctx));
   char* buf = nullptr;
-  int rc = mp_buffer(ctx, al256(sizeof(double) * (size_t)n) + al256(sizeof(unsigned) * tri_chain_ctl_words(n)) + 256,
-                     &buf);
+  const size_t xb = al256(sizeof(double) * (size_t)n);
+  int rc = mp_buffer(ctx, 2 * xb + al256(sizeof(unsigned) * tri_chain_ctl_words(1)) + 256, &buf);
   if (rc) return rc;
   double* d_x = reinterpret_cast<double*>(buf);
-  unsigned* ctl = reinterpret_cast<unsigned*>(buf + al256(sizeof(double) * (size_t)n));
-  int* err = reinterpret_cast<int*>(buf + al256(sizeof(double) * (size_t)n) +
-                                    al256(sizeof(unsigned) * tri_chain_ctl_words(n)));
+  double* d_y = reinterpret_cast<double*>(buf + xb);
+  unsigned* ctl = reinterpret_cast<unsigned*>(buf + 2 * xb);
+  int* err = reinterpret_cast<int*>(buf + 2 * xb + al256(sizeof(unsigned) * tri_chain_ctl_words(1)));
   const int kase1 = onenrm ? 1 : 2;
   std::vector<double> v(n), x(n);
   std::vector<int> isgn(n);
@@ -849,11 +849,11 @@ int gecon_device(bool onenrm, int n, const double* AF, long ldaf, double anorm, 
     if (kase == 0) break;
     MP_CK(hipMemcpyAsync(d_x, x.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     if (kase == kase1) {
-      launch_tri_chain<double>(0, n, 1, AF, ldaf, d_x, n, ctl, err, s);  // inv(L)
-      launch_tri_chain<double>(1, n, 1, AF, ldaf, d_x, n, ctl, err, s);  // inv(U)
+      launch_tri_chain<double>(0, n, 1, AF, ldaf, d_x, n, d_y, n, ctl, err, s);  // inv(L)
+      launch_tri_chain<double>(1, n, 1, AF, ldaf, d_y, n, d_x, n, ctl, err, s);  // inv(U)
     } else {
-      launch_tri_chain<double>(2, n, 1, AF, ldaf, d_x, n, ctl, err, s);  // inv(U^T)
-      launch_tri_chain<double>(3, n, 1, AF, ldaf, d_x, n, ctl, err, s);  // inv(L^T)
+      launch_tri_chain<double>(2, n, 1, AF, ldaf, d_x, n, d_y, n, ctl, err, s);  // inv(U^T)
+      launch_tri_chain<double>(3, n, 1, AF, ldaf, d_y, n, d_x, n, ctl, err, s);  // inv(L^T)
     }
     int e = 0;
     MP_CK(hipMemcpyAsync(x.data(), d_x, sizeof(double) * n, hipMemcpyDeviceToHost, s));

@@ -4,19 +4,20 @@
 //
 // lapack_amd_dgetrs_noalloc_async runs the same kernels as
 // lapack_amd_dgetrs_async (the chain form of DGETRS, SRC/dgetrs.f:181-218),
-// but its workspace -- the row permutation, the permuted right-hand sides,
-// the sweeps' ticket and flag words, the wait-error word and the composed
-// interchange pairs -- lives in `scratch`, a file-scope device array of
-// LA_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends).  Host code
-// never passes the array's name to a kernel: the first call on a device
-// takes the array's device address with hipGetSymbolAddress (with that
-// device current) and keeps it per device.  Every call on one GPU shares
-// that GPU's array, so calls that overlap on one GPU (on different streams,
-// or from different threads without ordering) race; calls on one stream are
-// ordered.  The scratch needs no clearing between calls: every word a call
-// reads it has written itself first (the ticket and flag words are zeroed
-// by a fill kernel before each sweep).  No hipMalloc, hipFree or allocating
-// library routine is called.
+// but its workspace -- the row permutation, the two n x nrhs buffers the
+// out-of-place sweeps alternate between, the sweeps' ticket words, the
+// wait-error word and the composed interchange pairs -- lives in `scratch`,
+// a file-scope device array of LA_NOALLOC_SCRATCH_BYTES per GPU (held until
+// the process ends).  Host code never passes the array's name to a kernel:
+// the first call on a device takes the array's device address with
+// hipGetSymbolAddress (with that device current) and keeps it per device.
+// Every call on one GPU shares that GPU's array, so calls that overlap on
+// one GPU (on different streams, or from different threads without
+// ordering) race; calls on one stream are ordered.  The scratch needs no
+// clearing between calls: every word a call reads it has written itself
+// first (before each sweep, fill kernels zero its ticket words and set its
+// output to the sentinel the sweep polls for).  No hipMalloc, hipFree or
+// allocating library routine is called.
 #include "la_common.h"
 
 #include <atomic>
