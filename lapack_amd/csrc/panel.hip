@@ -349,6 +349,11 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
       for (int c = 0; c < W; ++c) v[c] = T(0);
     }
   }
+  // the row's loads complete here, before the column loop: left pending,
+  // they would make the compiler wait for every outstanding memory
+  // operation (this column's exchange stores included) wherever the loop
+  // first reads v, such as the candidate row's publication
+  wait_vm();
   bool failed = false;
   int rotated = 0;
   for (int g = 0; g < 8 && 8 * g < kmax; ++g) {
@@ -378,8 +383,7 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
           if (kx > bk) { bk = kx; bw = x; }
         }
         if (wave == bw) {
-          // header first: the readers' first poll needs only the keys, the
-          // row is fetched by their second
+          // header first, then the row (readers check every word's flag)
           u64* sl = buf + (size_t)b * LL_SLOT;
           if (lane == 0) {
             st_dev(sl + 0, ll_word((unsigned)(bk & 0xffffffffull), flag));
@@ -392,14 +396,20 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
         // the pivot row needs it, once the exchange is over, so it stays off
         // the path to the workgroup's header (measured: publishing it first
         // made the workgroup holding the panel's top rows the last to
-        // publish in every column)
+        // publish in every column, 117.5 -> 121.5 ms at n = 16384)
         if (k >= wrow0 && k < wrow0 + 64) publish_row(buf + (size_t)LL_SMAX * LL_SLOT, k - wrow0, flag, v);
-        // 3. wave 0: the G candidates, then the winner's row and row k
+        // 3. wave 0: the G candidates, the winner's row and row k, in one
+        // polling loop: with every poll it also loads row k and the row of
+        // the best candidate seen so far (max key, lowest slot -- the final
+        // rule once every header is in), so when the last header does not
+        // change the winner, its row is already there
         if (wave == 0) {
           u64 kv = 0ull;
           bool kok = lane >= G;
-          double rkv = 0.0;
-          bool rk_ok = false;
+          double rkv = 0.0, pv = 0.0;
+          bool rk_ok = false, ok = false, rok = false;
+          u64 roww = 0;
+          int spec = -1;
           const u64* rks = buf + (size_t)LL_SMAX * LL_SLOT;
           unsigned spins = 0;
           while (!failed) {
@@ -411,29 +421,22 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
                 kok = true;
               }
             }
-            if (__all(kok)) break;
-            __builtin_amdgcn_s_sleep(1);
-            ++spins;
-            if ((spins & 255u) == 1u && ld_dev(err) != 0) failed = true;
-            if (spins > (1u << 20)) { if (lane == 0) st_dev(err, 1); failed = true; }
-          }
-          const u64 gmax = wave_max_u64(kv);
-          int gs = (int)__builtin_ctzll(__ballot(kv == gmax));
-          if (gs >= G) gs = 0;
-          const u64* src = buf + (size_t)gs * LL_SLOT;
-          double pv = 0.0;
-          bool ok = false;
-          u64 roww = 0;
-          bool rok = false;
-          spins = 0;
-          while (!failed) {
+            if (!rk_ok) rk_ok = ll_get_double(rks + 4 + 2 * lane, flag, &rkv);
+            const u64 km = wave_max_u64(kv);
+            int best = (int)__builtin_ctzll(__ballot(kv == km));
+            if (best >= G) best = 0;
+            if (best != spec) {
+              spec = best;
+              ok = false;
+              rok = false;
+            }
+            const u64* src = buf + (size_t)spec * LL_SLOT;
             if (!ok) ok = ll_get_double(src + 4 + 2 * lane, flag, &pv);
             if (!rok) {
               roww = ld_dev(src + 2);
               rok = (unsigned)(roww >> 32) == flag;
             }
-            if (!rk_ok) rk_ok = ll_get_double(rks + 4 + 2 * lane, flag, &rkv);
-            if (__all(ok && rok && rk_ok)) break;
+            if (__all(kok && ok && rok && rk_ok)) break;
             __builtin_amdgcn_s_sleep(1);
             ++spins;
             if ((spins & 255u) == 1u && ld_dev(err) != 0) failed = true;
