@@ -327,13 +327,24 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
   const int kmax = min(h, w);
   const int wrow0 = b * NTH + wave * 64;
   const T sfmin = Sfmin<T>::v;
-  auto publish_row = [&](u64* dst, int src, unsigned fl, const T* vals) {
+  // Publishes the row held by lane src.  With a pending (deferred) update of
+  // the previous column -- multiplier lp of the source row, pivot row pr --
+  // the columns c with cmin < c < cmax are published as updated, so readers
+  // get the row as the reference has it at this column.
+  auto publish_row = [&](u64* dst, int src, unsigned fl, const T* vals, bool pend, T lp, bool ap, const T* pr,
+                         int cmin, int cmax) {
     if (lane == src) {
 #pragma unroll
       for (int c = 0; c < W; ++c) s_pub[wave][c] = vals[c];
     }
     __builtin_amdgcn_wave_barrier();
-    ll_put_double(dst + 4 + 2 * lane, (double)s_pub[wave][lane], fl);
+    T val = s_pub[wave][lane];
+    if (pend) {
+      const T ls = __shfl(lp, src);
+      const bool as = __shfl((int)ap, src) != 0;
+      if (as && lane > cmin && lane < cmax) val = val - ls * pr[lane];
+    }
+    ll_put_double(dst + 4 + 2 * lane, (double)val, fl);
     __builtin_amdgcn_wave_barrier();
   };
 
@@ -356,6 +367,14 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
   wait_vm();
   bool failed = false;
   int rotated = 0;
+  // look-ahead: after column k's exchange a row updates only column k+1 (the
+  // next pivot column) before the next pivot search; the rest of column k's
+  // update is deferred until this workgroup has published column k+1's
+  // candidate, so it runs while the exchange is in flight.  Not across the
+  // end of a group (the rotation) nor after the leaf's last column.
+  bool pend = false;   // a deferred update of the previous column is due (uniform)
+  T l_prev = T(0);     // its multiplier for this thread's row
+  bool act_prev = false;
   for (int g = 0; g < 8 && 8 * g < kmax; ++g) {
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) {
@@ -390,14 +409,29 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
             st_dev(sl + 1, ll_word((unsigned)(bk >> 32), flag));
             st_dev(sl + 2, ll_word((unsigned)s_wrow[bw], flag));
           }
-          if (bk != 0ull) publish_row(sl, wl, flag, v);
+          if (bk != 0ull) publish_row(sl, wl, flag, v, pend, l_prev, act_prev, s_prow[par ^ 1], kk, 8 * (8 - g));
         }
         // row k, for the interchange, after the candidate: only the owner of
         // the pivot row needs it, once the exchange is over, so it stays off
         // the path to the workgroup's header (measured: publishing it first
         // made the workgroup holding the panel's top rows the last to
         // publish in every column, 117.5 -> 121.5 ms at n = 16384)
-        if (k >= wrow0 && k < wrow0 + 64) publish_row(buf + (size_t)LL_SMAX * LL_SLOT, k - wrow0, flag, v);
+        if (k >= wrow0 && k < wrow0 + 64)
+          publish_row(buf + (size_t)LL_SMAX * LL_SLOT, k - wrow0, flag, v, pend, l_prev, act_prev, s_prow[par ^ 1], kk,
+                      8 * (8 - g));
+        // the previous column's deferred update, while the exchange is in
+        // flight (its pivot row stays in s_prow[par ^ 1] until barrier 2)
+        if (pend && act_prev) {
+#pragma unroll
+          for (int c = kk + 1; c < 8; ++c) v[c] = v[c] - l_prev * s_prow[par ^ 1][c];
+#pragma unroll
+          for (int q = 1; q < 8; ++q) {
+            if (q < 8 - g) {
+#pragma unroll
+              for (int c = 8 * q; c < 8 * q + 8; ++c) v[c] = v[c] - l_prev * s_prow[par ^ 1][c];
+            }
+          }
+        }
         // 3. wave 0: the G candidates, the winner's row and row k, in one
         // polling loop: with every poll it also loads row k and the row of
         // the best candidate seen so far (max key, lowest slot -- the final
@@ -463,20 +497,30 @@ __global__ __launch_bounds__(NTH) void getf2_panel_w8_kernel(T* __restrict__ Tm,
 #pragma unroll
           for (int c = 0; c < W; ++c) v[c] = s_rowk[par][c];
         }
-        if (lr > k && lr < h) {
+        // defer the rest of this column's update when column k+1 is in this
+        // group and is not the leaf's last column's successor
+        const bool defer = kk < 7 && k + 1 < kmax;
+        act_prev = lr > k && lr < h;
+        if (act_prev) {
           T l = v[kk];
           if (piv != T(0)) l = (fabs(piv) >= sfmin) ? (T(1) / piv) * l : l / piv;
           v[kk] = l;
+          l_prev = l;
+          if (defer) {
+            if (kk < 7) v[kk + 1] = v[kk + 1] - l * s_prow[par][kk + 1];
+          } else {
 #pragma unroll
-          for (int c = kk + 1; c < 8; ++c) v[c] = v[c] - l * s_prow[par][c];
+            for (int c = kk + 1; c < 8; ++c) v[c] = v[c] - l * s_prow[par][c];
 #pragma unroll
-          for (int q = 1; q < 8; ++q) {
-            if (q < 8 - g) {
+            for (int q = 1; q < 8; ++q) {
+              if (q < 8 - g) {
 #pragma unroll
-              for (int c = 8 * q; c < 8 * q + 8; ++c) v[c] = v[c] - l * s_prow[par][c];
+                for (int c = 8 * q; c < 8 * q + 8; ++c) v[c] = v[c] - l * s_prow[par][c];
+              }
             }
           }
         }
+        pend = defer;
       }
     }
     {
