@@ -102,8 +102,15 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
     for (int k0 = 0; k0 < n; k0 += NB) {
       const int w = min(NB, n - k0);
       const int mr = n - k0;
-      for (int r = tid; r < mr; r += NT)
-        for (int c = 0; c < w; ++c) P[c * ldp + r] = A[(k0 + r) + (long)(k0 + c) * lda];
+      // the panel into LDS: a row's NB loads issued together
+      for (int r = tid; r < mr; r += NT) {
+        T pv[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) pv[c] = c < w ? A[(k0 + r) + (long)(k0 + c) * lda] : T(0);
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c < w) P[c * ldp + r] = pv[c];
+      }
       __syncthreads();
       // ---- panel in LDS.  Per column: the keys of column j were taken at
       // the end of the previous column's update (or just below for j = 0);
@@ -217,27 +224,43 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
       // ---- columns to the right, BC at a time
       const int c1 = k0 + w;
       for (int cb = c1; cb < n; cb += BC) {
-        if (tid < BC) {
-          const int c = cb + tid;
-          if (c < n) {
-            T* col = A + (long)c * lda;
-            T x[NB];
+        {
+          // U12 = L11^-1 A12 for the block's BC = NT/4 columns, four threads
+          // per column, thread sub of a column holding rows sub, sub+4, ...;
+          // x(j) passes from its owner to the other three by a lane shuffle.
+          // Per element the same operations in the same order as the
+          // reference's loop (dtrsm.f:289-300, zero test included).
+          constexpr int NQ = (NB + 3) / 4;
+          const int cl = tid >> 2, sub = tid & 3;
+          const int c = cb + cl;
+          const bool cok = c < n;
+          T* col = A + (long)(cok ? c : 0) * lda;
+          T xs[NQ];
 #pragma unroll
-            for (int j = 0; j < NB; ++j) x[j] = (j < w) ? col[k0 + j] : T(0);
+          for (int m = 0; m < NQ; ++m) {
+            const int i = sub + 4 * m;
+            xs[m] = (cok && i < w) ? col[k0 + i] : T(0);
+          }
 #pragma unroll
-            for (int j = 0; j < NB; ++j) {
-              if (j < w && x[j] != T(0)) {
+          for (int j = 0; j < NB; ++j) {
+            if (j < w) {
+              const T xj = __shfl(xs[j >> 2], (tid & 63 & ~3) | (j & 3));
+              if (xj != T(0)) {
 #pragma unroll
-                for (int i = j + 1; i < NB; ++i)
-                  if (i < w) x[i] = x[i] - x[j] * P[j * ldp + i];
+                for (int m = 0; m < NQ; ++m) {
+                  const int i = sub + 4 * m;
+                  if (i > j && i < w) xs[m] = xs[m] - xj * P[j * ldp + i];
+                }
               }
             }
+          }
 #pragma unroll
-            for (int j = 0; j < NB; ++j)
-              if (j < w) { col[k0 + j] = x[j]; U[j * BC + tid] = x[j]; }
-          } else {
-#pragma unroll
-            for (int j = 0; j < NB; ++j) U[j * BC + tid] = T(0);
+          for (int m = 0; m < NQ; ++m) {
+            const int i = sub + 4 * m;
+            if (i < NB) {
+              if (cok && i < w) col[k0 + i] = xs[m];
+              U[i * BC + cl] = (cok && i < w) ? xs[m] : T(0);
+            }
           }
         }
         __syncthreads();  // U complete
@@ -252,6 +275,16 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
             for (int q = 0; q < TN; ++q) acc[i][q] = T(0);
           if (r0 < n) {
             const int nrow = min(TM, n - r0);
+            // the tile's C values first, so their latency runs under the
+            // products below
+            T cv[TM][TN];
+#pragma unroll
+            for (int q = 0; q < TN; ++q) {
+              const int c = cb + cc + q;
+              const T* col = A + (long)(c < n ? c : 0) * lda;
+#pragma unroll
+              for (int i = 0; i < TM; ++i) cv[i][q] = (c < n && i < nrow) ? col[r0 + i] : T(0);
+            }
             for (int j = 0; j < w; ++j) {
               T a[TM], u[TN];
 #pragma unroll
@@ -270,7 +303,7 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
                 T* col = A + (long)c * lda;
 #pragma unroll
                 for (int i = 0; i < TM; ++i)
-                  if (r0 + i < n) col[r0 + i] = col[r0 + i] - acc[i][q];
+                  if (i < nrow) col[r0 + i] = cv[i][q] - acc[i][q];
               }
             }
           }
