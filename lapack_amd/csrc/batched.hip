@@ -204,12 +204,10 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
       }
       __syncthreads();
       const int np = s_np;
-      // ---- panel back to HBM; interchanges on every other column
-      for (int r = tid; r < mr; r += NT)
-        for (int c = 0; c < w; ++c) A[(k0 + r) + (long)(k0 + c) * lda] = P[c * ldp + r];
-      if (np > 0) {
-        for (int cc2 = tid; cc2 < n - w; cc2 += NT) {
-          const int c = cc2 < k0 ? cc2 : cc2 + w;
+      // interchanges on columns [c_lo, c_hi): a column per thread, its loads
+      // issued together, then its stores
+      auto swap_cols = [&](int c_lo, int c_hi) {
+        for (int c = c_lo + tid; c < c_hi; c += NT) {
           T* col = A + (long)c * lda;
           T tmp[2 * NB];
 #pragma unroll
@@ -219,7 +217,11 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
           for (int q = 0; q < 2 * NB; ++q)
             if (q < np) col[s_ppos[q]] = tmp[q];
         }
-      }
+      };
+      // ---- the interchanges on the columns right of the panel, which the
+      // solves and the update below read; the panel's write-back and the
+      // columns left of it wait until after the update
+      if (np > 0) swap_cols(k0 + w, n);
       barrier_vm();  // the solves below read rows other threads moved
       // ---- columns to the right, BC at a time
       const int c1 = k0 + w;
@@ -310,6 +312,10 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
         }
         __syncthreads();
       }
+      // ---- panel back to HBM; the interchanges on the columns left of it
+      for (int r = tid; r < mr; r += NT)
+        for (int c = 0; c < w; ++c) A[(k0 + r) + (long)(k0 + c) * lda] = P[c * ldp + r];
+      if (np > 0) swap_cols(0, k0);
       barrier_vm();  // the next panel loads what this one's update wrote
     }
     if (tid == 0) infov[b] = s_info;
