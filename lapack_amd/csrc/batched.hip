@@ -81,7 +81,6 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
   __shared__ u64 s_key[NW];
   __shared__ int s_row[NW];
   __shared__ int s_piv[NB];
-  __shared__ T s_pv;  // the pivot value of the current column
   __shared__ int s_ppos[2 * NB], s_psrc[2 * NB];  // the panel's interchanges as (position, source) pairs
   __shared__ int s_np;
   __shared__ int s_item, s_info;
@@ -126,37 +125,42 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
         wave_argmax(key, row);
         if ((tid & 63) == 0) { s_key[tid >> 6] = key; s_row[tid >> 6] = row; }
         __syncthreads();
+        // every thread folds the waves' candidates (first max, lowest row)
+        u64 bk = s_key[0];
+        int br = s_row[0];
+        for (int q = 1; q < NW; ++q)
+          if (s_key[q] > bk || (s_key[q] == bk && s_row[q] < br)) { bk = s_key[q]; br = s_row[q]; }
+        if (br >= mr || br < j) br = j;  // cannot happen (row j is always a candidate); keeps indexes in range
+        const int p = br;
+        // the pivot value and A(j, j), read before anything rewrites column
+        // j: the interchange below leaves column j alone, and the update
+        // writes it only after that interchange's barrier (or, without an
+        // interchange, only below row j = p)
+        const T pv = P[j * ldp + p];
+        const T ajj = P[j * ldp + j];
         if (tid == 0) {
-          u64 bk = s_key[0];
-          int br = s_row[0];
-          for (int q = 1; q < NW; ++q)
-            if (s_key[q] > bk || (s_key[q] == bk && s_row[q] < br)) { bk = s_key[q]; br = s_row[q]; }
-          if (br >= mr || br < j) br = j;  // cannot happen (row j is always a candidate); keeps indexes in range
-          s_piv[j] = br;
-          s_pv = P[j * ldp + br];
-          ipiv[k0 + j] = k0 + br + 1;
-          if (s_pv == T(0) && s_info == 0) s_info = k0 + j + 1;
+          s_piv[j] = p;
+          ipiv[k0 + j] = k0 + p + 1;
+          if (pv == T(0) && s_info == 0) s_info = k0 + j + 1;
         }
-        __syncthreads();
-        // The pivot value comes from LDS written before the barrier: the
-        // interchange below rewrites P(p, j) while other waves may not yet
-        // have read it.
-        const int p = s_piv[j];
-        const T pv = s_pv;
-        if (pv != T(0) && p != j) {
+        const bool swp = pv != T(0) && p != j;
+        if (swp) {
           for (int c = tid; c < w; c += NT) {
-            const T t = P[c * ldp + j];
-            P[c * ldp + j] = P[c * ldp + p];
-            P[c * ldp + p] = t;
+            if (c != j) {
+              const T t = P[c * ldp + j];
+              P[c * ldp + j] = P[c * ldp + p];
+              P[c * ldp + p] = t;
+            }
           }
           __syncthreads();
+          if (tid == 0) P[j * ldp + j] = pv;  // column j's half of the interchange
         }
         const bool recip = fabs(pv) >= sfmin;
         const T rcp = T(1) / pv;
         key = 0;
         row = 0x7fffffff;
         for (int r = j + 1 + tid; r < mr; r += NT) {
-          T l = P[j * ldp + r];
+          T l = (swp && r == p) ? ajj : P[j * ldp + r];
           if (pv != T(0)) {
             l = recip ? rcp * l : l / pv;
             P[j * ldp + r] = l;
