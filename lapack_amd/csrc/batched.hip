@@ -57,6 +57,25 @@ __device__ __forceinline__ void wave_argmax(u64& key, int& row) {
   }
 }
 
+// FP64 / FP32 16x16x4 matrix-core product on gfx950: lane l holds A(l%16,
+// l/16) and B(l/16, l%16) of a k-step; accumulator register r of lane l
+// holds D(row(l/16, r), l%16).
+template <class T> struct BMfma;
+template <> struct BMfma<double> {
+  typedef la_d4 acc_t;
+  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+};
+template <> struct BMfma<float> {
+  typedef la_f4 acc_t;
+  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
 constexpr int TM = 4, TN = 4, TX = 16;
 
 // Barrier after global-memory writes that other waves of the workgroup read
@@ -67,7 +86,7 @@ __device__ __forceinline__ void barrier_vm() {
 }
 
 template <class T, int NB, int NT>
-__global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int* __restrict__ nv,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NB <= 16 ? 2 : 1))) void getrf_batched_kernel(int batch, const int* __restrict__ nv,
                                                            T* const* __restrict__ av, const int* __restrict__ ldav,
                                                            int* const* __restrict__ ipv, int* __restrict__ infov,
                                                            const int* __restrict__ order, unsigned* counter, int ldp) {
@@ -265,51 +284,64 @@ __global__ __launch_bounds__(NT) void getrf_batched_kernel(int batch, const int*
             const int i = sub + 4 * m;
             if (i < NB) {
               if (cok && i < w) col[k0 + i] = xs[m];
-              U[i * BC + cl] = (cok && i < w) ? xs[m] : T(0);
+              U[i * BC + cl] = (cok && i < w) ? -xs[m] : T(0);  // negated: the update adds
             }
           }
         }
         __syncthreads();  // U complete
-        const int tx = tid % TX, ty = tid / TX;
-        const int cc = ty * TN;  // tile column offset within the block
-        for (int rb = c1; rb < n; rb += BR) {
-          const int r0 = rb + tx * TM;  // absolute row
-          T acc[TM][TN];
+        // ---- A22 -= A21*U12 (dgetrf.f:212-216) on the matrix cores, 64-row
+        // blocks: wave (wr, wc) owns rows 32 wr .. +31 and columns 32 wc ..
+        // +31 of the block (2 x 2 MFMA blocks).  Issued as D^T = U^T A21^T
+        // with the accumulator starting from C and U holding -U12, so lane
+        // l%16 walks rows and C moves in whole 128-byte pieces; C is loaded
+        // before the products.
+        {
+          typedef BMfma<T> MM;
+          typedef typename MM::acc_t acc_t;
+          const int lane = tid & 63, wave = tid >> 6;
+          const int li = lane & 15, lk = lane >> 4;
+          const int wr = wave & 1, wc = wave >> 1;
+          const int nks = (w + 3) >> 2;
+          for (int rb = c1; rb < n; rb += 64) {
+            const int row0 = rb + 32 * wr;
+            acc_t acc[2][2];
 #pragma unroll
-          for (int i = 0; i < TM; ++i)
+            for (int bi = 0; bi < 2; ++bi) {
+              const int row = row0 + 16 * bi + li;
 #pragma unroll
-            for (int q = 0; q < TN; ++q) acc[i][q] = T(0);
-          if (r0 < n) {
-            const int nrow = min(TM, n - r0);
-            // the tile's C values first, so their latency runs under the
-            // products below
-            T cv[TM][TN];
+              for (int bj = 0; bj < 2; ++bj) {
 #pragma unroll
-            for (int q = 0; q < TN; ++q) {
-              const int c = cb + cc + q;
-              const T* col = A + (long)(c < n ? c : 0) * lda;
-#pragma unroll
-              for (int i = 0; i < TM; ++i) cv[i][q] = (c < n && i < nrow) ? col[r0 + i] : T(0);
+                for (int r = 0; r < 4; ++r) {
+                  const int col = cb + 32 * wc + 16 * bj + MM::row(lk, r);
+                  acc[bi][bj][r] = (row < n && col < n) ? A[row + (long)col * lda] : T(0);
+                }
+              }
             }
-            for (int j = 0; j < w; ++j) {
-              T a[TM], u[TN];
+            for (int s4 = 0; s4 < nks; ++s4) {
+              const int k = 4 * s4 + lk;
+              T af[2], bf[2];
 #pragma unroll
-              for (int i = 0; i < TM; ++i) a[i] = (i < nrow) ? P[j * ldp + (r0 - k0) + i] : T(0);
+              for (int bj = 0; bj < 2; ++bj) af[bj] = k < w ? U[k * BC + 32 * wc + 16 * bj + li] : T(0);
 #pragma unroll
-              for (int q = 0; q < TN; ++q) u[q] = U[j * BC + cc + q];
+              for (int bi = 0; bi < 2; ++bi) {
+                const int row = row0 + 16 * bi + li;
+                bf[bi] = (k < w && row < n) ? P[k * ldp + (row - k0)] : T(0);
+              }
 #pragma unroll
-              for (int i = 0; i < TM; ++i)
+              for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-                for (int q = 0; q < TN; ++q) acc[i][q] = fma(a[i], u[q], acc[i][q]);
+                for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = MM::op(af[bj], bf[bi], acc[bi][bj]);
             }
 #pragma unroll
-            for (int q = 0; q < TN; ++q) {
-              const int c = cb + cc + q;
-              if (c < n) {
-                T* col = A + (long)c * lda;
+            for (int bi = 0; bi < 2; ++bi) {
+              const int row = row0 + 16 * bi + li;
 #pragma unroll
-                for (int i = 0; i < TM; ++i)
-                  if (i < nrow) col[r0 + i] = cv[i][q] - acc[i][q];
+              for (int bj = 0; bj < 2; ++bj) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                  const int col = cb + 32 * wc + 16 * bj + MM::row(lk, r);
+                  if (row < n && col < n) A[row + (long)col * lda] = acc[bi][bj][r];
+                }
               }
             }
           }
