@@ -366,6 +366,33 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
   }
 }
 
+// The trailing update C <- C - A*B with direct global -> LDS loads
+// (buffer_load_dwordx4 ... lds: each lane moves 16 bytes into LDS, the wave's
+// 64 pieces land contiguously at a wave-uniform LDS address), for A, B, C
+// 16-byte aligned, M and the leading dimensions multiples of the 16-byte
+// vector and K a multiple of 16 (the launcher checks; other shapes keep
+// update_big_kernel).  Tile 256 x 128, 8 waves of 64 x 64 as above.
+//
+// No staging registers: a k stage of 16 goes straight to LDS, and every
+// fragment a wave needs from it is one 16-byte LDS read:
+//   * lane (li, lk) takes k = 4*lk + kk in MFMA step kk (any assignment of
+//     the 16 k of a stage to (lk, kk) is a valid product), so the four k of
+//     one lane are consecutive in B's column: B's stage is stored as 16-
+//     column blocks, piece p of lane L at block*256 + p*64*VE + L*VE, one
+//     contiguous 1 KB wave read per block and piece;
+//   * a lane's A fragments are VE consecutive rows of one k (rows VE*li+e of
+//     the half h), one read giving VE of the wave's four 16-row MFMA blocks;
+//     block b = h*VE + e then holds rows 16*VE*h + VE*li + e, so the
+//     accumulators of the blocks e = 0..VE-1 are VE consecutive rows of one
+//     column of C: C is read and written as 16-byte vectors.
+// Each stage's fragments are all read before the next stage's loads are
+// issued into the other buffer, so those loads get the whole MFMA phase to
+// arrive; one barrier per stage (after this wave's loads are complete)
+// publishes the stage.
+template <class T> struct V16;
+template <> struct V16<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct V16<double> { typedef double type __attribute__((ext_vector_type(2))); };
+typedef __attribute__((address_space(3))) void la_lds_void;
 // Buffer-resource access (MUBUF): a uniform 64-bit base in scalar registers
 // plus a 32-bit per-lane byte offset and a uniform byte offset, so a tile's
 // loads and stores keep one offset register per lane instead of one 64-bit
@@ -374,6 +401,151 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, 0x7fffffff, 0x00020000);
 }
+// 16 bytes per lane from buffer r at byte offset voff into LDS: the wave's
+// 64 pieces land contiguously from the wave-uniform address lds.  (A plain
+// function: with the builtin written inside a kernel template the host
+// compilation pass drops that kernel's launch stub.)
+__device__ __forceinline__ void lds_dma16(__amdgpu_buffer_rsrc_t r, void* lds, int voff) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (la_lds_void*)lds, 16, voff, 0, 0, 0);
+}
+
+template <class T>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+void update_dma_kernel(int M, int N, int K, const T* __restrict__ A, long lda, const T* __restrict__ B, long ldb,
+                       T* __restrict__ C, long ldc, int tm, int tn, int chunk, int nxcd) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  typedef typename V16<T>::type v_t;
+  constexpr int VE = 16 / (int)sizeof(T);  // elements per 16-byte piece
+  constexpr int NP = 4 / VE;               // pieces per lane and k stage (FP32 1, FP64 2)
+  constexpr int TBM = 256, TBN = 128, KS = 16;
+  constexpr int ASZ = KS * TBM, BSZ = TBN * KS;  // elements of one stage
+  constexpr int CPR = TBM * (int)sizeof(T) / 1024;  // wave loads per k row of A
+  constexpr int NLA = ASZ * (int)sizeof(T) / 1024 / 8, NLB = BSZ * (int)sizeof(T) / 1024 / 8;  // per wave
+  __shared__ __align__(16) T As[2][ASZ];
+  __shared__ __align__(16) T Bs[2][BSZ];
+
+  const int bid = blockIdx.x;
+  int xcd = bid % NXCD;
+  if (nxcd < NXCD) {
+    if (xcd < NXCD - nxcd) return;
+    xcd -= NXCD - nxcd;
+  }
+  const int per = gridDim.x / NXCD;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave & 3, wn = wave >> 2;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nkt = K / KS;
+#pragma nounroll
+  for (int slot = bid / NXCD; slot < chunk; slot += per) {
+    const int lin = __builtin_amdgcn_readfirstlane(xcd * chunk + slot);
+    if (lin >= tm * tn) break;
+    const int width = GROUP_M * tn;
+    const int group = lin / width;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(tm - first_m, GROUP_M);
+    const int row0 = (first_m + (lin % width) % gsize) * TBM;
+    const int col0 = ((lin % width) / gsize) * TBN;
+
+    // this wave's loads of stage kt into buffer buf, as buffer loads from
+    // the stage's corner (uniform) plus a byte offset per lane fixed for the
+    // tile (below 2^31: the launcher checks 16*lda and 128*ldb): rows past M
+    // and columns past N read clamped in-bounds pieces (their products reach
+    // only rows and columns of C that are not stored)
+    int aoff[NLA], boff[NLB];
+#pragma unroll
+    for (int q = 0; q < NLA; ++q) {
+      const int a = wave + 8 * q, ka = a / CPR, c = a % CPR;
+      aoff[q] = (min(row0 + c * 64 * VE + VE * lane, M - VE) - row0 + ka * (int)lda) * (int)sizeof(T);
+    }
+#pragma unroll
+    for (int q = 0; q < NLB; ++q) {
+      const int b = wave + 8 * q, jb = b / NP, p = b % NP;
+      boff[q] = (4 * lk + VE * p + (min(col0 + 16 * jb + li, N - 1) - col0) * (int)ldb) * (int)sizeof(T);
+    }
+    auto issue = [&](int kt, int buf) {
+      const long k0 = (long)kt * KS;
+      const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + row0 + k0 * lda);
+      const __amdgpu_buffer_rsrc_t rb = buf_rsrc(B + k0 + (long)col0 * ldb);
+#pragma unroll
+      for (int q = 0; q < NLA; ++q) {
+        const int a = wave + 8 * q, ka = a / CPR, c = a % CPR;
+        lds_dma16(ra, &As[buf][ka * TBM + c * 64 * VE], aoff[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < NLB; ++q) {
+        const int b = wave + 8 * q, jb = b / NP, p = b % NP;
+        lds_dma16(rb, &Bs[buf][jb * 256 + p * 64 * VE], boff[q]);
+      }
+    };
+
+    // stage 0 first: the reads of C into the accumulators go out behind it
+    issue(0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    acc_t acc[4][4];
+    const int rw = row0 + wm * 64 + VE * li;  // + 16*VE*h
+    const int cw = col0 + wn * 64;            // + 16*jb + row(lk, r)
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const T* cp = C + (long)min(cw + 16 * jb + Mfma<T>::row(lk, r), N - 1) * ldc;
+#pragma unroll
+        for (int h = 0; h < NP; ++h) {
+          const v_t v = *(const v_t*)(cp + min(rw + 16 * VE * h, M - VE));
+#pragma unroll
+          for (int e = 0; e < VE; ++e) acc[h * VE + e][jb][r] = v[e];
+        }
+      }
+    for (int kt = 0; kt < nkt; ++kt) {
+      const int cur = kt & 1;
+      __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's loads of stage kt are in LDS
+      __syncthreads();
+      v_t af[4][NP], bf[4][NP];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+        for (int h = 0; h < NP; ++h)
+          af[kk][h] = *(const v_t*)&As[cur][(4 * lk + kk) * TBM + wm * 64 + h * 16 * VE + VE * li];
+#pragma unroll
+      for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+        for (int p = 0; p < NP; ++p) bf[jb][p] = *(const v_t*)&Bs[cur][(wn * 4 + jb) * 256 + p * 64 * VE + lane * VE];
+      if (kt + 1 < nkt) issue(kt + 1, cur ^ 1);
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+        for (int h = 0; h < NP; ++h)
+#pragma unroll
+          for (int e = 0; e < VE; ++e)
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb)
+              acc[h * VE + e][jb] = Mfma<T>::op(-bf[jb][kk / VE][kk % VE], af[kk][h][e], acc[h * VE + e][jb]);
+    }
+    // every wave is done with the last stage's buffer before the next tile's
+    // stage 0 is loaded (the same buffer when K/16 is odd)
+    __syncthreads();
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gc = cw + 16 * jb + Mfma<T>::row(lk, r);
+        if (gc >= N) continue;
+        T* cp = C + (long)gc * ldc;
+#pragma unroll
+        for (int h = 0; h < NP; ++h) {
+          const int gr = rw + 16 * VE * h;
+          if (gr >= M) continue;
+          v_t v;
+#pragma unroll
+          for (int e = 0; e < VE; ++e) v[e] = acc[h * VE + e][jb][r];
+          *(v_t*)(cp + gr) = v;
+        }
+      }
+  }
+}
+
 typedef int la_i2 __attribute__((ext_vector_type(2)));
 template <class T> __device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff);
 template <> __device__ __forceinline__ double buf_ld<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
@@ -684,6 +856,10 @@ __global__ __launch_bounds__(256) void gemm64_kernel(int M, int N, int K, T alph
 }
 
 template <class T>
+void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long ldb, T* C, long ldc, int nskip,
+                     hipStream_t s, bool alone = false);
+
+template <class T>
 void launch_gemm_t(char ta, char tb, int m, int n, int k, T alpha, const T* A, long lda, const T* B,
                    long ldb, T beta, T* C, long ldc, hipStream_t s, int nskip) {
   if (m <= 0 || n <= 0) return;
@@ -697,6 +873,12 @@ void launch_gemm_t(char ta, char tb, int m, int n, int k, T alpha, const T* A, l
   const bool TA = (ta == 'T' || ta == 't' || ta == 'C' || ta == 'c');
   const bool TB = (tb == 'T' || tb == 't' || tb == 'C' || tb == 'c');
   if (k <= 0 || alpha == T(0)) k = 0;  // C <- beta*C (dgemm.f:303-318)
+  // C - A*B, the trailing-update form: the update kernels (C read into the
+  // accumulators first, A negated)
+  if (!TA && !TB && nskip == 0 && k > 0 && alpha == T(-1) && beta == T(1)) {
+    launch_update_t<T>(m, n, k, A, lda, B, ldb, C, ldc, 0, s, true);
+    return;
+  }
   if (!TA && !TB && nskip == 0 && tiles < 64 && k >= 64) {
     const int tm64 = (m + 63) / 64, tn64 = (n + 63) / 64;
     gemm64_kernel<T><<<dim3(tm64 * tn64), 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm64);
@@ -772,10 +954,46 @@ static int update_pipe() {
   return (e && *e) ? std::atoi(e) : 0;
 }
 
+// Large updates on the direct-to-LDS kernel where its alignment and shape
+// conditions hold (LAPACK_AMD_UPD_DMA, default 1; 0 keeps them on
+// update_big_kernel).  Measured on MI355X, m = n = 16128, k = 256, alone:
+// FP64 47.8 -> 54.8 TFLOP/s, FP32 83.9 -> 119.6 (121.9 at m = n = 32512).
+// (read at every launch, so that a process may switch it)
+static int update_dma() {
+  const char* e = std::getenv("LAPACK_AMD_UPD_DMA");
+  return (e && *e) ? std::atoi(e) : 1;
+}
+// FP32 workgroups of the direct-to-LDS kernel per CU beside a panel
+// (LAPACK_AMD_UPD_DMA_OCC, default 1).  With 2, a CU freed by half is
+// refilled by the next tile before a panel workgroup (which needs more
+// registers than half a CU) fits: SGETRF at n = 32768 took 379 ms with 2
+// (panel chain 306 ms) against 321 ms with 1 (panel chain 146 ms).
+static int update_dma_occ() {
+  const char* e = std::getenv("LAPACK_AMD_UPD_DMA_OCC");
+  return (e && *e) ? std::atoi(e) : 1;
+}
+template <class T>
+static bool update_dma_fits(int m, int k, const T* A, long lda, const T* B, long ldb, const T* C, long ldc) {
+  constexpr long VE = 16 / sizeof(T);
+  const unsigned long al = (unsigned long)A | (unsigned long)B | (unsigned long)C;
+  return (al & 15) == 0 && m % VE == 0 && k % 16 == 0 && lda % VE == 0 && ldb % VE == 0 && ldc % VE == 0 &&
+         (16 * lda + 256) * (long)sizeof(T) < (1L << 31) && (128 * ldb + 16) * (long)sizeof(T) < (1L << 31);
+}
+
+// Tiles (256 x 128) from which an update takes the big kernels
+// (LAPACK_AMD_UPD_BIG_MIN, default 512; tests lower it to reach those
+// kernels at small orders).  Read at every launch.
+static long update_big_min() {
+  const char* e = std::getenv("LAPACK_AMD_UPD_BIG_MIN");
+  return (e && *e) ? std::atol(e) : 512;
+}
+
+// alone: no panel runs beside this update (a DGEMM call of the update form)
 template <class T>
 void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long ldb, T* C, long ldc, int nskip,
-                     hipStream_t s) {
+                     hipStream_t s, bool alone) {
   if (m <= 0 || n <= 0 || k <= 0) return;
+  const long big_min = update_big_min();
   const int nx = NXCD - nskip;
   // 256x128 tiles once they still give every CU two workgroups or more.
   // Beside a panel (nskip > 0) an update that is not much longer than the
@@ -784,7 +1002,7 @@ void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long
   // queueing behind the update's workgroups.
   const int tmb = (m + 255) / 256, tnb = (n + 127) / 128;
   const long ldmax = std::max(lda, std::max(ldb, ldc)) * (long)sizeof(T);
-  if (update_pipe() && (long)tmb * tnb >= 512 && ldmax < (1L << 24)) {
+  if (update_pipe() && (long)tmb * tnb >= big_min && ldmax < (1L << 24)) {
     // the 128 x 128 tiles inside C on the pipelined kernel, the edge strips
     // (fewer than 128 rows or columns) on gemm_kernel
     const int tmp = m / 128, tnp = n / 128;
@@ -810,10 +1028,18 @@ void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long
     }
     return;
   }
-  if ((long)tmb * tnb >= 512) {
+  if ((long)tmb * tnb >= big_min) {
     const int chunk = (tmb * tnb + nx - 1) / nx;
     int per = chunk;
     if (nskip > 0 && 2.0 * m * (double)n * k < update_persist_flops()) per = std::min(chunk, sizeof(T) == 4 ? update_cus_per_xcd_s() : update_cus_per_xcd());
+    if (update_dma() && update_dma_fits<T>(m, k, A, lda, B, ldb, C, ldc)) {
+      // FP32: 40 KB of dynamic LDS hold the kernel to one workgroup per CU
+      // (two fit otherwise), so a CU freed by a finished tile is free whole
+      // for the panel's workgroups instead of taking the next tile at once
+      const size_t dyn = (sizeof(T) == 4 && !alone && update_dma_occ() == 1) ? 40 * 1024 : 0;
+      update_dma_kernel<T><<<dim3(per * NXCD), 512, dyn, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tmb, tnb, chunk, nx);
+      return;
+    }
     update_big_kernel<T, 4, 2><<<dim3(per * NXCD), 512, 0, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tmb, tnb, chunk,
                                                                nx);
     return;

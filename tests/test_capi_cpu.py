@@ -32,6 +32,14 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_every_kernel_launch_stub_is_defined():
+    """Each kernel the host code launches has its launch stub in the library: a kernel template whose host
+    pass fails to instantiate leaves an undefined __device_stub__ symbol, which a shared library still links
+    with and which fails only when that kernel is first launched."""
+    out = os.popen(f"nm -C --undefined-only {la.LIB_PATH}").read()
+    assert "__device_stub__" not in out, [l for l in out.splitlines() if "__device_stub__" in l]
+
+
 def test_xerbla_is_weak_and_overridable():
     out = os.popen(f"nm -D {la.LIB_PATH} | grep ' xerbla_'").read()
     assert " W xerbla_" in out

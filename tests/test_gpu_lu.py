@@ -20,6 +20,7 @@ Bars (the reference's own test ratios):
   * INFO equal (exact zero pivots).
 """
 import numpy as np
+import torch
 import pytest
 
 import lapack_amd as la
@@ -124,6 +125,51 @@ def test_dgemm_dblat3_ratio(dev, ta, tb, m, n, k):
         assert rc == 0
         cc = from_dev(d_c, (m, n))
         assert dmmch_ratio(ta, tb, alpha, a, b, beta, c, cc, 2.0 ** -52) < 16.0
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["d", "s"])
+@pytest.mark.parametrize("m,n,k,pad,big_min", [
+    (1000, 1300, 64, 4, "1"),   # direct-to-LDS kernel: padded leading dimensions, edge tiles in M and N
+    (1001, 1300, 64, 0, "1"),   # M not a multiple of the 16-byte vector: the 256 x 128 register-staged kernel
+    (1000, 1300, 72, 0, "1"),   # K not a multiple of 16: the same
+    (4100, 4200, 64, 0, None),  # the default tile threshold, direct-to-LDS kernel
+])
+def test_gemm_update_form(dev, monkeypatch, dtype, m, n, k, pad, big_min):
+    """C - A*B (TRANSA = TRANSB = 'N', ALPHA = -1, BETA = 1: the trailing update of SRC/dgetrf.f:212-216)
+    on the update kernels, in the DBLAT3 ratio (BLAS/TESTING/dblat3.f:2940-2960) against the oracle's
+    DGEMM of the same (rounded) inputs, eps = 2^-52 or 2^-23."""
+    if big_min is not None:
+        monkeypatch.setenv("LAPACK_AMD_UPD_BIG_MIN", big_min)
+    rng = np.random.default_rng(m + 7 * n + 31 * k + pad)
+    a = rng.uniform(-1, 1, (m, k)).astype(dtype)
+    b = rng.uniform(-1, 1, (k, n)).astype(dtype)
+    c = rng.uniform(-1, 1, (m, n)).astype(dtype)
+    lda, ldb, ldc = m + pad, k + pad, m + pad
+    ap = np.zeros((lda, k), dtype, order="F"); ap[:m] = a
+    bp = np.zeros((ldb, n), dtype, order="F"); bp[:k] = b
+    cp = np.zeros((ldc, n), dtype, order="F"); cp[:m] = c
+    d_a, d_b, d_c = to_dev(ap), to_dev(bp), to_dev(cp)
+    if dtype == np.float64:
+        assert la.dgemm("N", "N", m, n, k, -1.0, d_a, lda, d_b, ldb, 1.0, d_c, ldc) == 0
+    else:
+        assert la.sgemm_async("N", "N", m, n, k, -1.0, d_a, lda, d_b, ldb, 1.0, d_c, ldc) == 0
+        torch.cuda.synchronize()
+    cc = from_dev(d_c, (ldc, n))
+    assert np.array_equal(cc[m:], cp[m:]), "rows past M were written"
+    eps = 2.0 ** -52 if dtype == np.float64 else 2.0 ** -23
+    r = dmmch_ratio("N", "N", -1.0, a.astype(np.float64), b.astype(np.float64), 1.0, c.astype(np.float64),
+                    cc[:m].astype(np.float64), eps)
+    assert r < 16.0, r
+
+
+@pytest.mark.parametrize("occ", ["1", "2"])
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_sgetrf_parity_big_update_kernels(dev, monkeypatch, n, occ):
+    """SGETRF with every trailing update that fits on the direct-to-LDS kernel (tile threshold 1), one
+    or two workgroups per CU; IPIV against the reference SGETRF as test_sgetrf_parity."""
+    monkeypatch.setenv("LAPACK_AMD_UPD_BIG_MIN", "1")
+    monkeypatch.setenv("LAPACK_AMD_UPD_DMA_OCC", occ)
+    test_sgetrf_parity(dev, n)
 
 
 def test_dgemm_beta_zero_does_not_read_c(dev):

@@ -6,7 +6,12 @@ inputs as the default forms:
     software-pipelined 128 x 128 kernel (edge strips on the 128 x 128
     one-tile kernel) instead of the 256 x 128 one-tile kernel;
   * LAPACK_AMD_PANEL_XCD=0: the multi-workgroup panel shares every XCD with
-    the update instead of owning XCD 0.
+    the update instead of owning XCD 0;
+  * LAPACK_AMD_UPD_DMA=0: large trailing updates on the register-staged
+    256 x 128 kernel instead of the direct-to-LDS one;
+  * LAPACK_AMD_UPD_BIG_MIN=1: every trailing update that fits on the
+    direct-to-LDS kernel (persistent beside the panel), at orders where the
+    default threshold would leave them on the 128 x 128 kernels.
 """
 import numpy as np
 import pytest
@@ -35,8 +40,9 @@ def _solve(a, b):
     return lu, p, x
 
 
-@pytest.mark.parametrize("env", [{"LAPACK_AMD_UPD_PIPE": "1"}, {"LAPACK_AMD_PANEL_XCD": "0"}],
-                         ids=["upd_pipe", "panel_shared"])
+@pytest.mark.parametrize("env", [{"LAPACK_AMD_UPD_PIPE": "1"}, {"LAPACK_AMD_PANEL_XCD": "0"},
+                                 {"LAPACK_AMD_UPD_DMA": "0"}, {"LAPACK_AMD_UPD_BIG_MIN": "1"}],
+                         ids=["upd_pipe", "panel_shared", "upd_staged", "upd_dma_all"])
 @pytest.mark.parametrize("n,nrhs", [(3000, 16), (4100, 5)])
 def test_variant_against_oracle(dev, monkeypatch, env, n, nrhs):
     for k, v in env.items():
