@@ -613,7 +613,8 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": ("DGETRF trailing update A22 -= A21*U12: update_big_kernel<double,4,2> (256x128 tiles), "
+            "kernel": ("DGETRF trailing update A22 -= A21*U12: update_dma_kernel<double> (256x128 tiles, "
+                       "direct global-to-LDS loads), "
                        "gemm64_kernel<double,true> (64x64 tiles) for the next panel's columns, "
                        "gemm_kernel<double,N,N,update> (128x128) for the last updates beside the panel"),
             "achieved": round(achieved, 3),
