@@ -21,6 +21,8 @@
 // scattered 4-byte pieces of 16 columns.
 #include "la_common.h"
 
+#include <cstdlib>
+
 namespace la {
 namespace {
 
@@ -186,14 +188,178 @@ __global__ __launch_bounds__(HNT, 2) void gemm_update_h_kernel(int M, int N, int
   }
 }
 
+// The same update for large updates with the operand feed of the FP32
+// direct-to-LDS kernel (update_dma_kernel<float> in gemm.hip): 256 x 128
+// tile, 8 waves of 64 x 64, each k stage of 16 moved from HBM straight into
+// LDS as FP32 (buffer_load_dwordx4 ... lds), double-buffered, one barrier per
+// stage.  The operands are rounded to the half format when read:
+//   * B (the block of L): lane (li, lk) reads the four consecutive k
+//     4*lk .. 4*lk+3 of column 16*jb + li as one 16-byte LDS read: exactly its
+//     four k slots of v_mfma_f32_16x16x16_f16 for the whole stage;
+//   * A: four 16-byte reads (k rows 4*lk + q, q = 0..3) give rows 4*li + e,
+//     e = 0..3, of the four MFMA blocks (block e holds rows 4*li + e), so
+//     component e of the four reads is block e's operand.
+// Accumulator r of lane l of block (e, jb) is C(4*(l%16) + e, 16*jb +
+// 4*(l/16) + r): the four blocks e of one (jb, r) are four consecutive rows,
+// so C is read and written as 16-byte vectors.  As in gemm_update_h_kernel,
+// products accumulate from zero and C - acc * 2^-12 (FP16) or C - acc (BF16)
+// is stored.  Conditions as update_dma_kernel: A, B, C 16-byte aligned, M and
+// the leading dimensions multiples of 4, K of 16 (the launcher checks).
+typedef __attribute__((address_space(3))) void h_lds_void;
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+typedef _Float16 h4v_t __attribute__((ext_vector_type(4)));
+typedef __bf16 b4v_t __attribute__((ext_vector_type(4)));
+// range never checked (num_records = 2^31 - 1): every offset stays inside
+// the operand's own matrix
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t hbuf_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, 0x7fffffff, 0x00020000);
+}
+// 16 bytes per lane into LDS (a plain function: written inside a kernel
+// template the builtin makes the host pass drop the kernel's launch stub)
+__device__ __forceinline__ void hlds_dma16(__amdgpu_buffer_rsrc_t r, void* lds, int voff) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (h_lds_void*)lds, 16, voff, 0, 0, 0);
+}
+template <int KIND> __device__ __forceinline__ unsigned long long half4_bits(f4v_t v);
+template <> __device__ __forceinline__ unsigned long long half4_bits<1>(f4v_t v) {
+  return __builtin_bit_cast(unsigned long long, __builtin_convertvector(v, h4v_t));
+}
+template <> __device__ __forceinline__ unsigned long long half4_bits<2>(f4v_t v) {
+  return __builtin_bit_cast(unsigned long long, __builtin_convertvector(v, b4v_t));
+}
+
+template <int KIND>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+void update_dma_h_kernel(int M, int N, int K, const float* __restrict__ A, long lda, const float* __restrict__ B,
+                         long ldb, float* __restrict__ C, long ldc, int tm, int tn, int chunk, int nxcd) {
+  constexpr float BSCALE = (KIND == 1) ? 4096.0f : 1.0f;
+  constexpr int TBM = 256, TBN = 128, KS = 16;
+  constexpr int ASZ = KS * TBM, BSZ = TBN * KS;
+  __shared__ __align__(16) float As[2][ASZ];
+  __shared__ __align__(16) float Bs[2][BSZ];
+
+  const int bid = blockIdx.x;
+  int xcd = bid % HNXCD;
+  if (nxcd < HNXCD) {
+    if (xcd < HNXCD - nxcd) return;
+    xcd -= HNXCD - nxcd;
+  }
+  const int per = gridDim.x / HNXCD;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave & 3, wn = wave >> 2;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nkt = K / KS;
+#pragma nounroll
+  for (int slot = bid / HNXCD; slot < chunk; slot += per) {
+    const int lin = __builtin_amdgcn_readfirstlane(xcd * chunk + slot);
+    if (lin >= tm * tn) break;
+    const int width = HGROUP_M * tn;
+    const int group = lin / width;
+    const int first_m = group * HGROUP_M;
+    const int gsize = min(tm - first_m, HGROUP_M);
+    const int row0 = (first_m + (lin % width) % gsize) * TBM;
+    const int col0 = ((lin % width) / gsize) * TBN;
+
+    // this wave's loads: k rows wave and wave + 8 of A (256 rows each, four
+    // per lane), and 16-column block wave of B (four k per lane); rows past
+    // M and columns past N read clamped in-bounds pieces that reach only C
+    // elements which are not stored
+    int aoff[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      aoff[q] = (min(row0 + 4 * lane, M - 4) - row0 + (wave + 8 * q) * (int)lda) * (int)sizeof(float);
+    const int boff = (4 * lk + (min(col0 + 16 * wave + li, N - 1) - col0) * (int)ldb) * (int)sizeof(float);
+    auto issue = [&](int kt, int buf) {
+      const long k0 = (long)kt * KS;
+      const __amdgpu_buffer_rsrc_t ra = hbuf_rsrc(A + row0 + k0 * lda);
+      const __amdgpu_buffer_rsrc_t rb = hbuf_rsrc(B + k0 + (long)col0 * ldb);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) hlds_dma16(ra, &As[buf][(wave + 8 * q) * TBM], aoff[q]);
+      hlds_dma16(rb, &Bs[buf][wave * 256], boff);
+    };
+    issue(0, 0);
+    f4_t acc[4][4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int jb = 0; jb < 4; ++jb) acc[e][jb] = f4_t{0, 0, 0, 0};
+    for (int kt = 0; kt < nkt; ++kt) {
+      const int cur = kt & 1;
+      __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's loads of stage kt are in LDS
+      __syncthreads();
+      f4v_t af[4], bf[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) af[q] = *(const f4v_t*)&As[cur][(4 * lk + q) * TBM + wm * 64 + 4 * li];
+#pragma unroll
+      for (int jb = 0; jb < 4; ++jb) bf[jb] = *(const f4v_t*)&Bs[cur][(wn * 4 + jb) * 256 + lane * 4];
+      if (kt + 1 < nkt) issue(kt + 1, cur ^ 1);
+      unsigned long long ah[4], bh[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) ah[e] = half4_bits<KIND>(f4v_t{af[0][e], af[1][e], af[2][e], af[3][e]});
+#pragma unroll
+      for (int jb = 0; jb < 4; ++jb) bh[jb] = half4_bits<KIND>(bf[jb] * BSCALE);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) acc[e][jb] = mfma_h<KIND>(bh[jb], ah[e], acc[e][jb]);
+    }
+    // every wave is done with the last stage's buffer before the next tile's
+    // stage 0 is loaded (the same buffer when K/16 is odd)
+    __syncthreads();
+    const int rw = row0 + wm * 64 + 4 * li;
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb) {
+      f4v_t cv[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        cv[r] = *(const f4v_t*)(C + (long)min(col0 + wn * 64 + 16 * jb + 4 * lk + r, N - 1) * ldc + min(rw, M - 4));
+      if (rw >= M) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gc = col0 + wn * 64 + 16 * jb + 4 * lk + r;
+        if (gc >= N) continue;
+        f4v_t v;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = cv[r][e] - acc[e][jb][r] * (1.0f / BSCALE);
+        *(f4v_t*)(C + (long)gc * ldc + rw) = v;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 // kind: 1 = FP16 operands, 2 = BF16 operands (FP32 accumulation either way).
 void launch_sgemm_update_half(int kind, int m, int n, int k, const float* A, long lda, const float* B, long ldb,
                               float* C, long ldc, int nskip, hipStream_t s) {
   if (m <= 0 || n <= 0 || k <= 0) return;
-  const int tm = (m + HBM_ - 1) / HBM_, tn = (n + HBN - 1) / HBN;
   const int nx = HNXCD - nskip;
+  // large updates on the direct-to-LDS kernel where it fits (the thresholds
+  // and switches of the FP32 update: LAPACK_AMD_UPD_DMA, _UPD_BIG_MIN,
+  // _UPD_DMA_OCC); beside a panel one workgroup per CU (40 KB of dynamic LDS)
+  {
+    const char* e = std::getenv("LAPACK_AMD_UPD_DMA");
+    const bool dma = !(e && *e && std::atoi(e) == 0);
+    const char* eb = std::getenv("LAPACK_AMD_UPD_BIG_MIN");
+    const long big_min = (eb && *eb) ? std::atol(eb) : 512;
+    const char* eo = std::getenv("LAPACK_AMD_UPD_DMA_OCC");
+    const int occ = (eo && *eo) ? std::atoi(eo) : 1;
+    const int tmb = (m + 255) / 256, tnb = (n + 127) / 128;
+    const unsigned long al = (unsigned long)A | (unsigned long)B | (unsigned long)C;
+    const bool fits = (al & 15) == 0 && m % 4 == 0 && k % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 &&
+                      (16 * lda + 256) * 4L < (1L << 31) && (128 * ldb + 16) * 4L < (1L << 31);
+    if (dma && fits && (long)tmb * tnb >= big_min) {
+      const int chunk = (tmb * tnb + nx - 1) / nx;
+      const size_t dyn = occ == 1 ? 40 * 1024 : 0;
+      if (kind == 2)
+        update_dma_h_kernel<2><<<dim3(chunk * HNXCD), 512, dyn, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tmb, tnb, chunk, nx);
+      else
+        update_dma_h_kernel<1><<<dim3(chunk * HNXCD), 512, dyn, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tmb, tnb, chunk, nx);
+      return;
+    }
+  }
+  const int tm = (m + HBM_ - 1) / HBM_, tn = (n + HBN - 1) / HBN;
   const int chunk = (tm * tn + nx - 1) / nx;
   if (kind == 2)
     gemm_update_h_kernel<2><<<dim3(chunk * HNXCD), HNT, 0, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tm, tn, chunk, nx);
