@@ -409,8 +409,11 @@ __device__ __forceinline__ void lds_dma16(__amdgpu_buffer_rsrc_t r, void* lds, i
   __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (la_lds_void*)lds, 16, voff, 0, 0, 0);
 }
 
-template <class T>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+// NSUB: k sub-stages of 16 per LDS stage.  With 2 (FP32 only: 96 KB of LDS,
+// one workgroup per CU) each stage's loads have two sub-stages of MFMAs to
+// arrive, and there is one barrier per 32 k.
+template <class T, int NSUB>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && NSUB == 1) ? 4 : 2)))
 void update_dma_kernel(int M, int N, int K, const T* __restrict__ A, long lda, const T* __restrict__ B, long ldb,
                        T* __restrict__ C, long ldc, int tm, int tn, int chunk, int nxcd) {
   typedef typename Mfma<T>::acc_t acc_t;
@@ -421,8 +424,8 @@ void update_dma_kernel(int M, int N, int K, const T* __restrict__ A, long lda, c
   constexpr int ASZ = KS * TBM, BSZ = TBN * KS;  // elements of one stage
   constexpr int CPR = TBM * (int)sizeof(T) / 1024;  // wave loads per k row of A
   constexpr int NLA = ASZ * (int)sizeof(T) / 1024 / 8, NLB = BSZ * (int)sizeof(T) / 1024 / 8;  // per wave
-  __shared__ __align__(16) T As[2][ASZ];
-  __shared__ __align__(16) T Bs[2][BSZ];
+  __shared__ __align__(16) T As[2][NSUB * ASZ];
+  __shared__ __align__(16) T Bs[2][NSUB * BSZ];
 
   const int bid = blockIdx.x;
   int xcd = bid % NXCD;
@@ -436,7 +439,7 @@ void update_dma_kernel(int M, int N, int K, const T* __restrict__ A, long lda, c
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave & 3, wn = wave >> 2;
   const int li = lane & 15, lk = lane >> 4;
-  const int nkt = K / KS;
+  const int nkt = K / (KS * NSUB);
 #pragma nounroll
   for (int slot = bid / NXCD; slot < chunk; slot += per) {
     const int lin = __builtin_amdgcn_readfirstlane(xcd * chunk + slot);
@@ -448,7 +451,7 @@ void update_dma_kernel(int M, int N, int K, const T* __restrict__ A, long lda, c
     const int row0 = (first_m + (lin % width) % gsize) * TBM;
     const int col0 = ((lin % width) / gsize) * TBN;
 
-    // this wave's loads of stage kt into buffer buf, as buffer loads from
+    // this wave's loads of stage kt (sub-stage su: k from (kt*NSUB+su)*16) into buffer buf, as buffer loads from
     // the stage's corner (uniform) plus a byte offset per lane fixed for the
     // tile (below 2^31: the launcher checks 16*lda and 128*ldb): rows past M
     // and columns past N read clamped in-bounds pieces (their products reach
@@ -465,18 +468,21 @@ void update_dma_kernel(int M, int N, int K, const T* __restrict__ A, long lda, c
       boff[q] = (4 * lk + VE * p + (min(col0 + 16 * jb + li, N - 1) - col0) * (int)ldb) * (int)sizeof(T);
     }
     auto issue = [&](int kt, int buf) {
-      const long k0 = (long)kt * KS;
-      const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + row0 + k0 * lda);
-      const __amdgpu_buffer_rsrc_t rb = buf_rsrc(B + k0 + (long)col0 * ldb);
 #pragma unroll
-      for (int q = 0; q < NLA; ++q) {
-        const int a = wave + 8 * q, ka = a / CPR, c = a % CPR;
-        lds_dma16(ra, &As[buf][ka * TBM + c * 64 * VE], aoff[q]);
-      }
+      for (int su = 0; su < NSUB; ++su) {
+        const long k0 = ((long)kt * NSUB + su) * KS;
+        const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + row0 + k0 * lda);
+        const __amdgpu_buffer_rsrc_t rb = buf_rsrc(B + k0 + (long)col0 * ldb);
 #pragma unroll
-      for (int q = 0; q < NLB; ++q) {
-        const int b = wave + 8 * q, jb = b / NP, p = b % NP;
-        lds_dma16(rb, &Bs[buf][jb * 256 + p * 64 * VE], boff[q]);
+        for (int q = 0; q < NLA; ++q) {
+          const int a = wave + 8 * q, ka = a / CPR, c = a % CPR;
+          lds_dma16(ra, &As[buf][su * ASZ + ka * TBM + c * 64 * VE], aoff[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < NLB; ++q) {
+          const int b = wave + 8 * q, jb = b / NP, p = b % NP;
+          lds_dma16(rb, &Bs[buf][su * BSZ + jb * 256 + p * 64 * VE], boff[q]);
+        }
       }
     };
 
@@ -502,26 +508,33 @@ void update_dma_kernel(int M, int N, int K, const T* __restrict__ A, long lda, c
       const int cur = kt & 1;
       __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's loads of stage kt are in LDS
       __syncthreads();
-      v_t af[4][NP], bf[4][NP];
+      v_t af[NSUB][4][NP], bf[NSUB][4][NP];
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
+      for (int su = 0; su < NSUB; ++su) {
 #pragma unroll
-        for (int h = 0; h < NP; ++h)
-          af[kk][h] = *(const v_t*)&As[cur][(4 * lk + kk) * TBM + wm * 64 + h * 16 * VE + VE * li];
+        for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
-      for (int jb = 0; jb < 4; ++jb)
+          for (int h = 0; h < NP; ++h)
+            af[su][kk][h] = *(const v_t*)&As[cur][su * ASZ + (4 * lk + kk) * TBM + wm * 64 + h * 16 * VE + VE * li];
 #pragma unroll
-        for (int p = 0; p < NP; ++p) bf[jb][p] = *(const v_t*)&Bs[cur][(wn * 4 + jb) * 256 + p * 64 * VE + lane * VE];
+        for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+          for (int p = 0; p < NP; ++p)
+            bf[su][jb][p] = *(const v_t*)&Bs[cur][su * BSZ + (wn * 4 + jb) * 256 + p * 64 * VE + lane * VE];
+      }
       if (kt + 1 < nkt) issue(kt + 1, cur ^ 1);
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
+      for (int su = 0; su < NSUB; ++su)
 #pragma unroll
-        for (int h = 0; h < NP; ++h)
+        for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
-          for (int e = 0; e < VE; ++e)
+          for (int h = 0; h < NP; ++h)
 #pragma unroll
-            for (int jb = 0; jb < 4; ++jb)
-              acc[h * VE + e][jb] = Mfma<T>::op(-bf[jb][kk / VE][kk % VE], af[kk][h][e], acc[h * VE + e][jb]);
+            for (int e = 0; e < VE; ++e)
+#pragma unroll
+              for (int jb = 0; jb < 4; ++jb)
+                acc[h * VE + e][jb] =
+                    Mfma<T>::op(-bf[su][jb][kk / VE][kk % VE], af[su][kk][h][e], acc[h * VE + e][jb]);
     }
     // every wave is done with the last stage's buffer before the next tile's
     // stage 0 is loaded (the same buffer when K/16 is odd)
@@ -968,6 +981,15 @@ static int update_dma() {
 // refilled by the next tile before a panel workgroup (which needs more
 // registers than half a CU) fits: SGETRF at n = 32768 took 379 ms with 2
 // (panel chain 306 ms) against 321 ms with 1 (panel chain 146 ms).
+// k per LDS stage of the FP32 direct-to-LDS kernel beside a panel
+// (LAPACK_AMD_UPD_DMA_KS, 16 or 32, default 32; read at every launch).
+// Alone 32 is 1-2 % slower (107.3 against 109.3 TFLOP/s at m = n = 16128),
+// beside the panel faster: SGETRF 95.2 -> 90.3 ms at n = 16384, 314.2 ->
+// 311.7 ms at n = 32768 (the panel's boundary launches wait less).
+static int update_dma_ks() {
+  const char* e = std::getenv("LAPACK_AMD_UPD_DMA_KS");
+  return (e && *e) ? std::atoi(e) : 32;
+}
 static int update_dma_occ() {
   const char* e = std::getenv("LAPACK_AMD_UPD_DMA_OCC");
   return (e && *e) ? std::atoi(e) : 1;
@@ -1036,8 +1058,17 @@ void launch_update_t(int m, int n, int k, const T* A, long lda, const T* B, long
       // FP32: 40 KB of dynamic LDS hold the kernel to one workgroup per CU
       // (two fit otherwise), so a CU freed by a finished tile is free whole
       // for the panel's workgroups instead of taking the next tile at once
-      const size_t dyn = (sizeof(T) == 4 && !alone && update_dma_occ() == 1) ? 40 * 1024 : 0;
-      update_dma_kernel<T><<<dim3(per * NXCD), 512, dyn, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tmb, tnb, chunk, nx);
+      const bool occ1 = sizeof(T) == 4 && !alone && update_dma_occ() == 1;
+      if constexpr (sizeof(T) == 4) {
+        if (occ1 && k % 32 == 0 && update_dma_ks() == 32) {
+          // FP32 beside a panel, stages of 32 k (96 KB of LDS: one workgroup per CU)
+          update_dma_kernel<T, 2><<<dim3(per * NXCD), 512, 0, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tmb, tnb, chunk,
+                                                                  nx);
+          return;
+        }
+      }
+      const size_t dyn = occ1 ? 40 * 1024 : 0;
+      update_dma_kernel<T, 1><<<dim3(per * NXCD), 512, dyn, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tmb, tnb, chunk, nx);
       return;
     }
     update_big_kernel<T, 4, 2><<<dim3(per * NXCD), 512, 0, s>>>(m, n, k, A, lda, B, ldb, C, ldc, tmb, tnb, chunk,

@@ -112,6 +112,24 @@ def test_ir_well_conditioned(dev, prec):
     print(f"prec {prec}: iter {it}, gmres {git}, berr {berr.max():.2e}, fwd {fwd:.2e}")
 
 
+@pytest.mark.parametrize("env", [{"LAPACK_AMD_UPD_BIG_MIN": "1"},
+                                 {"LAPACK_AMD_UPD_BIG_MIN": "1", "LAPACK_AMD_UPD_DMA": "0"}],
+                         ids=["direct_lds", "staged"])
+@pytest.mark.parametrize("prec", ["S", "H", "B"])
+def test_ir_big_update_kernels(dev, monkeypatch, env, prec):
+    """Every trailing update of the FP32-stored factorization that fits on the 256 x 128 kernels (tile
+    threshold 1): the direct global-to-LDS kernels (FP32, and FP16 / BF16 operands), or with
+    LAPACK_AMD_UPD_DMA=0 the register-staged ones; edge tiles in both dimensions at n = 1000."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    n, nrhs = 1000, 2
+    a = la.matrix(3, 720, n, n)
+    b = la.matrix(3, 721, n, nrhs)
+    x, it, git, info, berr = solve_ir(a, b, prec)
+    assert info == 0
+    check_solution(a, b, x, berr, it)
+
+
 @pytest.mark.parametrize("prec", ["S", "H", "B"])
 def test_ir_conditioned_1e6(dev, prec):
     """The conditioned variant of the mixed-precision workload (kappa = 1e6) at

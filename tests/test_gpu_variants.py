@@ -12,6 +12,9 @@ inputs as the default forms:
   * LAPACK_AMD_UPD_BIG_MIN=1: every trailing update that fits on the
     direct-to-LDS kernel (persistent beside the panel), at orders where the
     default threshold would leave them on the 128 x 128 kernels.
+The FP32 forms of the last two (and stages of 16 k instead of 32 for the
+FP32 direct-to-LDS kernel beside the panel) are run by
+test_sgetrf_variants_against_reference.
 """
 import numpy as np
 import pytest
@@ -57,3 +60,16 @@ def test_variant_against_oracle(dev, monkeypatch, env, n, nrhs):
     for trans in "NT":
         assert oracle.dget02(a, x[trans], b, trans) < THRESH, trans
     assert la.device_status(clear=True) == 0
+
+
+@pytest.mark.parametrize("env", [{"LAPACK_AMD_UPD_DMA": "0"}, {"LAPACK_AMD_UPD_BIG_MIN": "1"},
+                                 {"LAPACK_AMD_UPD_BIG_MIN": "1", "LAPACK_AMD_UPD_DMA_KS": "16"}],
+                         ids=["upd_staged", "upd_dma_all", "upd_dma_all_k16"])
+@pytest.mark.parametrize("n", [1500, 2048])
+def test_sgetrf_variants_against_reference(dev, monkeypatch, env, n):
+    """SGETRF under the update variants: IPIV against the reference SGETRF (near-tie bar in units of 2^-24)
+    and the SGET01-style ratio, as test_sgetrf_parity."""
+    from test_gpu_lu import test_sgetrf_parity
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    test_sgetrf_parity(dev, n)
