@@ -507,9 +507,9 @@ def main():
     tuned = None
     if not args.no_tune:
         # NB and leaf width from the library's autotuner at this order (the
-        # table is also written under gpurun_out/ when that folder exists)
-        out_dir = os.path.join(ROOT, "gpurun_out")
-        path = os.path.join(out_dir, "getrf_tune.json") if os.path.isdir(out_dir) else None
+        # table is also written to $LAPACK_AMD_ARTIFACT_DIR when that is set)
+        out_dir = os.environ.get("LAPACK_AMD_ARTIFACT_DIR", "")
+        path = os.path.join(out_dir, "getrf_tune.json") if out_dir and os.path.isdir(out_dir) else None
         print(f"[bench] autotuning NB / leaf at n={n}", file=sys.stderr, flush=True)
         t_nb, t_leaf, t_ms = la.tune_getrf(n, path)
         tuned = {"nb": t_nb, "leaf": t_leaf, "getrf_ms": round(t_ms, 3),

@@ -32,8 +32,9 @@ def test_reference_ge_suite_through_drop_in(dev):
         pytest.skip("reference test suite not built (needs the reference sources at build time)")
     r = subprocess.run([EXE], capture_output=True, text=True, timeout=600, cwd=os.path.dirname(EXE))
     out = r.stdout
-    log = os.path.join(ROOT, "gpurun_out")
-    if os.path.isdir(log):
+    # the suite's full output, when a folder for run artifacts is named
+    log = os.environ.get("LAPACK_AMD_ARTIFACT_DIR", "")
+    if log and os.path.isdir(log):
         with open(os.path.join(log, "xlinge.out"), "w") as f:
             f.write(out + r.stderr)
     assert r.returncode == 0, r.stderr[-2000:]
