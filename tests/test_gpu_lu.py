@@ -162,6 +162,41 @@ def test_gemm_update_form(dev, monkeypatch, dtype, m, n, k, pad, big_min):
     assert r < 16.0, r
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["d", "s"])
+@pytest.mark.parametrize("which", ["a", "b", "c"])
+def test_gemm_update_form_unaligned(dev, monkeypatch, dtype, which):
+    """C - A*B with one operand starting one element past a 16-byte boundary: the register-staged kernel
+    takes it (the direct-to-LDS kernel needs 16-byte aligned operands); DBLAT3 ratio as above."""
+    monkeypatch.setenv("LAPACK_AMD_UPD_BIG_MIN", "1")
+    m, n, k = 1000, 1300, 64
+    rng = np.random.default_rng(77 + ord(which))
+    a = rng.uniform(-1, 1, (m, k)).astype(dtype)
+    b = rng.uniform(-1, 1, (k, n)).astype(dtype)
+    c = rng.uniform(-1, 1, (m, n)).astype(dtype)
+
+    def dev_off(x, off):
+        t = torch.zeros(x.size + off, dtype=torch.float64 if dtype == np.float64 else torch.float32, device="cuda")
+        t[off:] = torch.from_numpy(np.asfortranarray(x).ravel(order="F")).cuda()
+        return t, t[off:]
+
+    offs = {"a": 0, "b": 0, "c": 0}
+    offs[which] = 1
+    ta, d_a = dev_off(a, offs["a"])
+    tb, d_b = dev_off(b, offs["b"])
+    tc, d_c = dev_off(c, offs["c"])
+    if dtype == np.float64:
+        assert la.dgemm("N", "N", m, n, k, -1.0, d_a, m, d_b, k, 1.0, d_c, m) == 0
+    else:
+        assert la.sgemm_async("N", "N", m, n, k, -1.0, d_a, m, d_b, k, 1.0, d_c, m) == 0
+        torch.cuda.synchronize()
+    cc = d_c.cpu().numpy().reshape(n, m).T
+    eps = 2.0 ** -52 if dtype == np.float64 else 2.0 ** -23
+    r = dmmch_ratio("N", "N", -1.0, a.astype(np.float64), b.astype(np.float64), 1.0, c.astype(np.float64),
+                    cc.astype(np.float64), eps)
+    assert r < 16.0, r
+    assert float(tc[0]) == 0.0 or offs["c"] == 0  # the element before C untouched
+
+
 @pytest.mark.parametrize("occ", ["1", "2"])
 @pytest.mark.parametrize("n", [1024, 2048])
 def test_sgetrf_parity_big_update_kernels(dev, monkeypatch, n, occ):
