@@ -623,49 +623,54 @@ __global__ __launch_bounds__(256) void trtri_kernel(int n, int nb, const T* __re
   T* Wb = W + (long)d * nb * ldw;  // W block d: nb x nb at column offset d*nb
   const int s = threadIdx.x;
   T x = (s == c) ? T(1) : T(0);
-  // the steps in groups of D, each group's triangle entries loaded first
+  // the steps in groups of D; each group's triangle entries are loaded while
+  // the group before it runs (two register sets in turn), so the loads wait
+  // behind D barrier steps instead of in front of them
   constexpr int D = 8;
+  T ua[D], ub[D];
+  // entries of the group starting at step t0 (descending for UPPER), read
+  // from clamped in-bounds addresses without a branch (so that the first use
+  // waits only for its own load); the update below uses an entry only for
+  // the rows it belongs to
+  const int sc = min(s, bs - 1);
+  auto load = [&](T (&u)[D], int t0) {
+#pragma unroll
+    for (int e = 0; e < D; ++e) {
+      const int t = min(max(UPPER ? t0 - e : t0 + e, 0), bs - 1);
+      u[e] = Ub[(long)sc + (long)t * ldu];
+    }
+  };
+  auto run = [&](const T (&u)[D], int t0) {
+#pragma unroll
+    for (int e = 0; e < D; ++e) {
+      const int t = UPPER ? t0 - e : t0 + e;
+      if (UPPER ? t < 0 : t >= bs) break;
+      if (s == t) {
+        if (!UNIT) x = x / Ub[(long)t + (long)t * ldu];
+        xs[t & 1] = x;
+      }
+      __syncthreads();
+      const T xt = xs[t & 1];
+      if (UPPER ? s < t : (s > t && s < bs)) x = x - xt * u[e];
+    }
+  };
   if (UPPER) {
-    for (int t0 = c; t0 >= 0; t0 -= D) {
-      T u[D];
-#pragma unroll
-      for (int e = 0; e < D; ++e) {
-        const int t = t0 - e;
-        u[e] = (t >= 0 && s < t) ? Ub[(long)s + (long)t * ldu] : T(0);
-      }
-#pragma unroll
-      for (int e = 0; e < D; ++e) {
-        const int t = t0 - e;
-        if (t < 0) break;
-        if (s == t) {
-          if (!UNIT) x = x / Ub[(long)t + (long)t * ldu];
-          xs[t & 1] = x;
-        }
-        __syncthreads();
-        const T xt = xs[t & 1];
-        if (s < t) x = x - xt * u[e];
-      }
+    load(ua, c);
+    for (int t0 = c; t0 >= 0; t0 -= 2 * D) {
+      load(ub, t0 - D);
+      run(ua, t0);
+      if (t0 - D < 0) break;
+      load(ua, t0 - 2 * D);
+      run(ub, t0 - D);
     }
   } else {
-    for (int t0 = c; t0 < bs; t0 += D) {
-      T u[D];
-#pragma unroll
-      for (int e = 0; e < D; ++e) {
-        const int t = t0 + e;
-        u[e] = (t < bs && s > t && s < bs) ? Ub[(long)s + (long)t * ldu] : T(0);
-      }
-#pragma unroll
-      for (int e = 0; e < D; ++e) {
-        const int t = t0 + e;
-        if (t >= bs) break;
-        if (s == t) {
-          if (!UNIT) x = x / Ub[(long)t + (long)t * ldu];
-          xs[t & 1] = x;
-        }
-        __syncthreads();
-        const T xt = xs[t & 1];
-        if (s > t && s < bs) x = x - xt * u[e];
-      }
+    load(ua, c);
+    for (int t0 = c; t0 < bs; t0 += 2 * D) {
+      load(ub, t0 + D);
+      run(ua, t0);
+      if (t0 + D >= bs) break;
+      load(ua, t0 + 2 * D);
+      run(ub, t0 + D);
     }
   }
   if (s < bs) Wb[(long)s + (long)c * ldw] = x;
