@@ -486,10 +486,17 @@ void update_dma_kernel(int M, int N, int K, const T* __restrict__ A, long lda, c
       }
     };
 
-    // stage 0 first: the reads of C into the accumulators go out behind it
+    // stage 0 first: the reads of C go out behind it.  With stages of 32 k
+    // (one workgroup per CU, nothing else to cover a tile's start) C is kept
+    // aside and added after the products (LATE_C: C + sum instead of the sum
+    // started from C), so its reads overlap the first stage instead of
+    // preceding it.
+    constexpr bool LATE_C = NSUB == 2;
+    constexpr int NCL = 4 * 4 * NP;  // this lane's reads of C
     issue(0, 0);
     __builtin_amdgcn_sched_barrier(0);
     acc_t acc[4][4];
+    v_t cpre[4][4][NP];
     const int rw = row0 + wm * 64 + VE * li;  // + 16*VE*h
     const int cw = col0 + wn * 64;            // + 16*jb + row(lk, r)
 #pragma unroll
@@ -500,13 +507,28 @@ void update_dma_kernel(int M, int N, int K, const T* __restrict__ A, long lda, c
 #pragma unroll
         for (int h = 0; h < NP; ++h) {
           const v_t v = *(const v_t*)(cp + min(rw + 16 * VE * h, M - VE));
+          if (LATE_C) {
+            cpre[jb][r][h] = v;
+          } else {
 #pragma unroll
-          for (int e = 0; e < VE; ++e) acc[h * VE + e][jb][r] = v[e];
+            for (int e = 0; e < VE; ++e) acc[h * VE + e][jb][r] = v[e];
+          }
         }
       }
+    if (LATE_C) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) acc[b][jb] = acc_t{};
+    }
+    static_assert(!LATE_C || NCL == 16, "vmcnt(16) below counts the reads of C");
     for (int kt = 0; kt < nkt; ++kt) {
       const int cur = kt & 1;
-      __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's loads of stage kt are in LDS
+      // this wave's loads of stage kt are in LDS: vmcnt(0), or at the first
+      // stage with LATE_C vmcnt(16) (the reads of C, issued after it, may
+      // still be in flight)
+      if (LATE_C && kt == 0) __builtin_amdgcn_s_waitcnt(0x4f70);
+      else __builtin_amdgcn_s_waitcnt(0x0f70);
       __syncthreads();
       v_t af[NSUB][4][NP], bf[NSUB][4][NP];
 #pragma unroll
@@ -552,7 +574,7 @@ void update_dma_kernel(int M, int N, int K, const T* __restrict__ A, long lda, c
           if (gr >= M) continue;
           v_t v;
 #pragma unroll
-          for (int e = 0; e < VE; ++e) v[e] = acc[h * VE + e][jb][r];
+          for (int e = 0; e < VE; ++e) v[e] = LATE_C ? cpre[jb][r][h][e] + acc[h * VE + e][jb][r] : acc[h * VE + e][jb][r];
           *(v_t*)(cp + gr) = v;
         }
       }
