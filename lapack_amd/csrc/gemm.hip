@@ -388,7 +388,9 @@ __global__ __launch_bounds__(WM * WN * 64) void update_big_kernel(int M, int N, 
 // Each stage's fragments are all read before the next stage's loads are
 // issued into the other buffer, so those loads get the whole MFMA phase to
 // arrive; one barrier per stage (after this wave's loads are complete)
-// publishes the stage.
+// publishes the stage.  A stage holds NSUB sub-stages of 16 k (NSUB = 2:
+// FP32 beside a panel, which also keeps C aside and adds it after the
+// products; see the kernel).
 template <class T> struct V16;
 template <> struct V16<float> { typedef float type __attribute__((ext_vector_type(4))); };
 template <> struct V16<double> { typedef double type __attribute__((ext_vector_type(2))); };
