@@ -601,6 +601,59 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(int nrows, int ncols, 
   }
 }
 
+// The same rows moved in place, one column at a time through LDS:
+// A(r,j) <- A_old(perm[r], j) for r < nrows.  The rows the permutation moves
+// are read into LDS and then written from their sources, so each column is
+// read and written once (a gather into scratch and a copy back read and write
+// it twice).  A fixed point r (perm[r] == r) is neither read nor written: a
+// bijection takes every source of a moved row from the moved rows.  Thread
+// tid holds perm of rows tid + 256u, u < U, in registers over all the columns
+// of its workgroup, and loads the next column while it writes this one.
+constexpr int PERMUTE_LDS_BYTES = 131072;
+
+template <class T, int U>
+__global__ __launch_bounds__(256) void permute_rows_lds_kernel(int nrows, int ncols, const int* __restrict__ perm,
+                                                               T* __restrict__ A, long lda) {
+  __shared__ __align__(16) T col[PERMUTE_LDS_BYTES / sizeof(T)];
+  const int tid = threadIdx.x;
+  int j = blockIdx.x;
+  if (j >= ncols) return;
+  int pr[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int r = tid + 256 * u;
+    pr[u] = r < nrows ? min(max(perm[r], 0), nrows - 1) : r;
+  }
+  T v[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int r = tid + 256 * u;
+    v[u] = pr[u] != r ? A[(long)r + (long)j * lda] : T(0);
+  }
+  for (; j < ncols; j += gridDim.x) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = tid + 256 * u;
+      if (pr[u] != r) col[r] = v[u];
+    }
+    __syncthreads();
+    const int jn = j + gridDim.x;
+    if (jn < ncols) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int r = tid + 256 * u;
+        v[u] = pr[u] != r ? A[(long)r + (long)jn * lda] : T(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = tid + 256 * u;
+      if (pr[u] != r) A[(long)r + (long)j * lda] = col[pr[u]];
+    }
+    __syncthreads();  // every read of col done before the next column's values go in
+  }
+}
+
 // ---- triangular block inverses -------------------------------------------
 // W = inv(tri(U)) for nb x nb blocks, nb <= 256; blockIdx.x = column c of W,
 // blockIdx.y = which diagonal block (batched along the diagonal with stride
@@ -834,6 +887,32 @@ void launch_gather_rows_s(int nrows, int ncols, const int* perm, const float* sr
   if (nrows > 0 && ncols > 0)
     gather_rows_kernel<float><<<grid_for((long)nrows * ncols), 256, 0, s>>>(nrows, ncols, perm, src, lds, dst, ldd,
                                                                             skip);
+}
+
+int permute_rows_max(int elem_bytes) { return PERMUTE_LDS_BYTES / elem_bytes; }
+
+template <class T>
+static bool launch_permute_rows_t(int nrows, int ncols, const int* perm, T* A, long lda, hipStream_t s) {
+  if (nrows <= 0 || ncols <= 0 || nrows > permute_rows_max((int)sizeof(T))) return false;
+  const int grid = std::min(ncols, 512);
+  if (nrows <= 2048) {
+    permute_rows_lds_kernel<T, 8><<<grid, 256, 0, s>>>(nrows, ncols, perm, A, lda);
+  } else if (nrows <= 4096) {
+    permute_rows_lds_kernel<T, 16><<<grid, 256, 0, s>>>(nrows, ncols, perm, A, lda);
+  } else if (nrows <= 8192) {
+    permute_rows_lds_kernel<T, 32><<<grid, 256, 0, s>>>(nrows, ncols, perm, A, lda);
+  } else if (nrows <= 16384) {
+    permute_rows_lds_kernel<T, 64><<<grid, 256, 0, s>>>(nrows, ncols, perm, A, lda);
+  } else {
+    if constexpr (sizeof(T) == 4) permute_rows_lds_kernel<T, 128><<<grid, 256, 0, s>>>(nrows, ncols, perm, A, lda);
+  }
+  return true;
+}
+bool launch_permute_rows_d(int nrows, int ncols, const int* perm, double* A, long lda, hipStream_t s) {
+  return launch_permute_rows_t<double>(nrows, ncols, perm, A, lda, s);
+}
+bool launch_permute_rows_s(int nrows, int ncols, const int* perm, float* A, long lda, hipStream_t s) {
+  return launch_permute_rows_t<float>(nrows, ncols, perm, A, lda, s);
 }
 
 template <class T>

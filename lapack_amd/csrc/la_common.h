@@ -88,6 +88,11 @@ void launch_swaps_to_perm(int nrows, int k1, int k2, const int* ipiv, int incx, 
                           int* gp = nullptr);
 size_t perm_pairs_ints(int k1, int k2);  // device scratch of launch_swaps_to_perm's two-launch form
 // skip (optional device int): nothing is written when *skip != 0.
+// A(r,j) <- A_old(perm[r], j) in place for r < nrows, one pass through LDS;
+// false (nothing launched) when nrows exceeds permute_rows_max(sizeof(T)).
+int permute_rows_max(int elem_bytes);
+bool launch_permute_rows_d(int nrows, int ncols, const int* perm, double* A, long lda, hipStream_t s);
+bool launch_permute_rows_s(int nrows, int ncols, const int* perm, float* A, long lda, hipStream_t s);
 void launch_gather_rows_d(int nrows, int ncols, const int* perm, const double* src, long lds, double* dst,
                           long ldd, hipStream_t s, const int* skip = nullptr);
 void launch_gather_rows_s(int nrows, int ncols, const int* perm, const float* src, long lds, float* dst,

@@ -327,6 +327,9 @@ template <> struct K<double> {
                      const int* skip = nullptr) {
     launch_gather_rows_d(nr, nc, perm, src, ls, dst, ld, s, skip);
   }
+  static void permute_rows(int nr, int nc, const int* perm, double* A, long lda, hipStream_t s) {
+    launch_permute_rows_d(nr, nc, perm, A, lda, s);
+  }
   static void trtri_blocks(bool up, bool unit, int n, int nb, const double* U, long ldu, double* W, long ldw,
                            hipStream_t s) {
     launch_trtri_blocks_d(up, unit, n, nb, U, ldu, W, ldw, s);
@@ -371,6 +374,9 @@ template <> struct K<float> {
   static void gather(int nr, int nc, const int* perm, const float* src, long ls, float* dst, long ld, hipStream_t s,
                      const int* skip = nullptr) {
     launch_gather_rows_s(nr, nc, perm, src, ls, dst, ld, s, skip);
+  }
+  static void permute_rows(int nr, int nc, const int* perm, float* A, long lda, hipStream_t s) {
+    launch_permute_rows_s(nr, nc, perm, A, lda, s);
   }
   static void trtri_blocks(bool up, bool unit, int n, int nb, const float* U, long ldu, float* W, long ldw,
                            hipStream_t s) {
@@ -889,20 +895,31 @@ int laswp_device(int ncols, T* A, long lda, int nrows, int k1, int k2, const int
   if (!c) return LA_ERR_NODEVICE;
   std::lock_guard<std::mutex> g(c->mu);
   if (ncols <= 0 || incx == 0 || k2 < k1 || nrows <= 0) return 0;
+  // The whole sequence is composed into one permutation (blocks of 256
+  // interchanges composed in parallel), which moves each column's rows in
+  // place through LDS in one pass; with more rows than LDS holds, the rows
+  // are gathered into scratch and copied back.
+  const bool inplace = nrows <= permute_rows_max((int)sizeof(T));
   char* ws = nullptr;
   Carve cv0;
   cv0.take<int>((size_t)nrows);
-  cv0.take<T>((size_t)nrows * ncols);
+  cv0.take<int>(perm_pairs_ints(k1 - 1, k2));
+  if (!inplace) cv0.take<T>((size_t)nrows * ncols);
   int rc = ws_acquire(c, cv0.off, s, &ws);
   if (rc) return rc;
   Carve cv;
   int* perm = reinterpret_cast<int*>(ws + cv.take<int>((size_t)nrows));
-  T* X = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nrows * ncols));
+  int* gpairs = reinterpret_cast<int*>(ws + cv.take<int>(perm_pairs_ints(k1 - 1, k2)));
   // k1, k2 are 1-based and inclusive in the reference; perm kernel takes
   // 0-based [k1-1, k2).
-  launch_swaps_to_perm(nrows, k1 - 1, k2, ipiv, incx, perm, s);
-  K<T>::gather(nrows, ncols, perm, A, lda, X, nrows, s);
-  K<T>::copy(nrows, ncols, X, nrows, A, lda, s);
+  launch_swaps_to_perm(nrows, k1 - 1, k2, ipiv, incx, perm, s, gpairs);
+  if (inplace) {
+    K<T>::permute_rows(nrows, ncols, perm, A, lda, s);
+  } else {
+    T* X = reinterpret_cast<T*>(ws + cv.take<T>((size_t)nrows * ncols));
+    K<T>::gather(nrows, ncols, perm, A, lda, X, nrows, s);
+    K<T>::copy(nrows, ncols, X, nrows, A, lda, s);
+  }
   return ws_release(c, s);
 }
 

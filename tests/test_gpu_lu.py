@@ -339,6 +339,26 @@ def test_dlaswp_bitwise(dev, incx):
     assert np.array_equal(aa, ref)
 
 
+@pytest.mark.parametrize("m,n,form,incx", [(5003, 700, "getrf", 1), (5003, 700, "any", 1), (5003, 700, "any", -1),
+                                           (300, 1100, "getrf", -1), (17000, 3, "any", 1)])
+def test_dlaswp_large_bitwise(dev, m, n, form, incx):
+    """DLASWP of a full sequence of M interchanges: the row count not a multiple
+    of 256, more columns than the in-place kernel's workgroups (each handles
+    several), targets at or after their step (DGETRF's form) or anywhere, and
+    more rows than one column in LDS holds (17000 FP64: gather and copy)."""
+    a = la.matrix(2, 77 + m + n, m, n)
+    rng = np.random.default_rng(m + n + incx)
+    k1, k2 = 1, m
+    if form == "getrf":
+        ipiv = np.array([rng.integers(k, m + 1) for k in range(1, m + 1)], dtype=np.int32)
+    else:
+        ipiv = rng.integers(1, m + 1, size=m).astype(np.int32)
+    ref = oracle.dlaswp(a, k1, k2, ipiv, incx)
+    aa = np.array(a, order="F")
+    assert la.dlaswp(n, aa, m, k1, k2, ipiv, incx) == 0
+    assert np.array_equal(aa, ref)
+
+
 @pytest.mark.parametrize("n", [64, 300, 1024])
 def test_sgetrf_parity(dev, n):
     a = la.matrix(2, 3000 + n, n, n).astype(np.float32)
